@@ -1,0 +1,228 @@
+#!/usr/bin/env python3
+"""FDK reconstruction benchmark on one MI355X.
+
+Workload (default): 1024^3 FP32 volume from 720 uint16 projections of 1024x1024
+pixels (0.4 mm pitch, 0.2 mm at the isocentre, SOD 500 mm -> FCD 2500 px),
+equiangular full circular scan, Shepp-Logan filter; phantom = 3-D Shepp-Logan plus
+64 seeded random ellipsoids, Poisson counts at I0 = 1e4.
+
+One step = the whole hot path on the GPU with the raw projections already in HBM:
+log + Feldkamp weight + ramp-family row filter (ctg_preprocess) and the voxel-driven
+backprojection of all 720 projections into the 1024^3 volume (ctg_backproject).
+value = voxel-updates (nx*ny*nz*P) / step time, in GUPS.  Also reported: the
+backprojection-only rate, the same step with FP16-stored filtered projections, the
+error of both against the FP64 GPU reconstruction, and the CPU oracle's
+backprojection rate on a bounded sub-slab of the same workload (reported only).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, 8 TB/s
+BYTES_PER_UPDATE = 8.0     # one FP32 voxel load + one store per (voxel, projection)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_workload(ctx, torch, args):
+    import ct_amd
+    from ct_amd.phantom import circular_angles, phantom_unit_radius, random_ellipsoids, shepp_logan_3d
+
+    W = H = args.det
+    P = args.nproj
+    n = args.grid
+    fcd = 500.0 / (0.4 * 500.0 / 1000.0)  # SOD / isocentre pitch = 2500 px
+    g = ct_amd.geometry_grid(W, H, fcd, 0.0, n, n, n)
+    ctx.set_geometry(g)
+    angles = circular_angles(P)
+    ctx.set_projections(angles)
+    unit = phantom_unit_radius(g.radius, H, fcd)
+    ell = shepp_logan_3d(unit, scale=1.0 / unit)
+    ell += [e[:9] + [e[9] * 0.1 / unit] for e in random_ellipsoids(64, 2, unit)]
+    raw = torch.empty((P, H, W), dtype=torch.uint16, device="cuda")
+    t0 = time.perf_counter()
+    ctx.phantom_images(ell, 0, P, 1.0e4, 65535.0, 2, raw, None)
+    torch.cuda.synchronize()
+    log(f"phantom: {P}x{H}x{W} uint16 in {time.perf_counter() - t0:.2f} s")
+    return g, raw, angles, 1.0e4 / 65535.0
+
+
+def cpu_baseline(g, filtered_rows, angles, seconds):
+    """Oracle backprojection (OpenMP, all host threads) on a central z-sub-slab."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as orc
+
+    og = orc.geometry_from(g)
+    P = filtered_rows.shape[0]
+
+    def run(nz):
+        sub = orc.geometry_from(g)
+        z0 = g.nz // 2 - nz // 2
+        sub.nz = nz
+        sub.z_pre = g.z_pre - z0
+        t0 = time.perf_counter()
+        orc.backproject_f32(sub, filtered_rows, angles)
+        return time.perf_counter() - t0
+
+    t1 = run(1)
+    nz = max(1, min(g.nz, int(seconds / max(t1, 1e-3))))
+    t = run(nz)
+    ups = g.nx * g.ny * nz * P
+    return {"value": ups / t / 1e9, "unit": "GUPS", "cores": orc.get_threads(), "kind": "port",
+            "sample": f"CPU oracle backprojection (reconstructionCore restatement, OpenMP) of {P} projections "
+                      f"into a {g.nx}x{g.ny}x{nz} central z-slab of the same grid, {t:.1f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--det", type=int, default=1024)
+    ap.add_argument("--nproj", type=int, default=720)
+    ap.add_argument("--grid", type=int, default=1024)
+    ap.add_argument("--no-precision", action="store_true", help="skip the FP64 reference and FP16 error")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--only", choices=["f32", "f16", "both"], default="both")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(FDK shards over z-slabs with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import ct_amd
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    info = ct_amd._ctg.device_info(0)
+    log(f"device: {info.name.decode()} CUs={info.compute_units} HBM={info.total_bytes / 2**30:.0f} GiB")
+    ctx = ct_amd.Context(0)
+    g, raw, angles, base = build_workload(ctx, torch, args)
+    P = args.nproj
+    updates = float(g.nx) * g.ny * g.nz * P
+    stream = torch.cuda.current_stream().cuda_stream
+    filt = "shepplogan"
+
+    def make_bufs(store):
+        return (torch.empty(ctx.filtered_bytes(store, P), dtype=torch.uint8, device=dev),
+                torch.empty(P, dtype=torch.float32, device=dev))
+
+    vol = torch.empty((g.nx, g.ny, g.nz), dtype=torch.float32, device=dev)
+
+    def run_store(store):
+        filtered, scales = make_bufs(store)
+        ev = []
+
+        def step():
+            ctx.preprocess(raw, "u16", P, base, "cpu", filt, store, filtered, scales, stream)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            ctx.backproject(filtered, store, scales, 0, P, vol, 0, g.nz, False, stream)
+            e1.record()
+            ev.append((e0, e1))
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        ev.clear()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / args.steps
+        bp = sum(a.elapsed_time(b) for a, b in ev) / len(ev) / 1e3
+        log(f"{store}: step {t * 1e3:.1f} ms ({updates / t / 1e9:.1f} GUPS), backprojection {bp * 1e3:.1f} ms "
+            f"({updates / bp / 1e9:.1f} GUPS)")
+        return t, bp, filtered, scales
+
+    res = {}
+    t32, bp32, filt32, sc32 = run_store("f32")
+    res["f32"] = (t32, bp32)
+    vol32 = vol.clone() if not args.no_precision else None
+    rows_for_cpu = None
+    if not args.no_cpu_baseline:
+        # filtered rows (first tap of each pair) for the CPU oracle
+        rows_for_cpu = filt32.view(torch.float32).view(P, g.det_width, g.det_height, 2)[..., 0] \
+            .transpose(1, 2).contiguous().cpu().numpy()
+    del filt32, sc32
+    if args.only in ("f16", "both"):
+        t16, bp16, f16buf, sc16 = run_store("f16")
+        res["f16"] = (t16, bp16)
+        del f16buf, sc16
+    vol16 = vol.clone() if not args.no_precision and "f16" in res else None
+
+    prec = {}
+    if not args.no_precision:
+        import numpy as np
+        vol64 = torch.empty((g.nx, g.ny, g.nz), dtype=torch.float64, device=dev)
+        t0 = time.perf_counter()
+        ctx.fdk_f64(raw, "u16", P, base, filt, vol64, 0, g.nz, stream)
+        torch.cuda.synchronize()
+        log(f"FP64 GPU reconstruction: {time.perf_counter() - t0:.1f} s")
+
+        def errs(v):
+            d = v.double() - vol64
+            rr = math.sqrt(float(torch.sum(d * d)) / float(torch.sum(vol64 * vol64)))
+            return rr, float(torch.max(torch.abs(d)))
+
+        prec["f32_rel_rmse_vs_f64"], prec["f32_maxabs_vs_f64"] = errs(vol32)
+        if vol16 is not None:
+            prec["f16_rel_rmse_vs_f64"], prec["f16_maxabs_vs_f64"] = errs(vol16)
+        del vol64
+
+    cpu = None
+    if rows_for_cpu is not None:
+        log("CPU baseline ...")
+        cpu = cpu_baseline(g, rows_for_cpu, angles, args.cpu_seconds)
+        log(f"CPU: {cpu['value']:.3f} GUPS on {cpu['cores']} threads")
+
+    t32, bp32 = res["f32"]
+    achieved = BYTES_PER_UPDATE * updates / bp32 / 1e9
+    out = {
+        "metric": "FDK voxel-update rate (backprojection GUPS), 1024^3 from 720x1024^2; FP32 vs FP16-stored projections",
+        "value": updates / t32 / 1e9,
+        "unit": "GUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t32 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (3-D Shepp-Logan + 64 seeded ellipsoids, Poisson I0=1e4, generated on the GPU)",
+        "config": {"workload": f"FDK {g.nx}x{g.ny}x{g.nz} FP32 from {P}x{g.det_height}x{g.det_width} uint16, "
+                               f"{filt}, full circular scan, FCD {g.fcd:.0f} px",
+                   "step": "preprocess (log, Feldkamp weight, row filter) + backprojection, inputs in HBM"},
+        "ms_per_recon_f32": t32 * 1e3,
+        "bp_ms_f32": bp32 * 1e3,
+        "bp_gups_f32": updates / bp32 / 1e9,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    if "f16" in res:
+        t16, bp16 = res["f16"]
+        out.update({"ms_per_recon_f16": t16 * 1e3, "bp_ms_f16": bp16 * 1e3, "bp_gups_f16": updates / bp16 / 1e9,
+                    "f16_over_f32_bp": bp32 / bp16, "f16_over_f32_step": t32 / t16})
+    out.update(prec)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

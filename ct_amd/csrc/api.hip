@@ -1,0 +1,648 @@
+// C ABI of libctg: context, geometry, stream-ordered operations and the z-slab
+// reconstruction pipeline.  See include/ctg.h for the contract of each entry point.
+#include "ctg_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+namespace ctg {
+
+namespace {
+thread_local std::string g_last_error;
+
+__global__ void fill_u32_kernel(uint32_t* d, uint32_t v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        d[i] = v;
+}
+
+// Per-image max |x| of row-major FP32 images (bits of a non-negative float order like
+// unsigned integers).
+__global__ void maxabs_kernel(const float* __restrict__ img, size_t per_image, unsigned* __restrict__ out) {
+    const size_t p = blockIdx.y;
+    const float* src = img + p * per_image;
+    float m = 0.0f;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < per_image; i += (size_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, fabsf(src[i]));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(&out[p], __float_as_uint(m));
+}
+}  // namespace
+
+ctg_status fail(ctg_status s, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return s;
+}
+
+void clear_error() { g_last_error.clear(); }
+
+ctg_status DevBuf::reserve(size_t n) {
+    if (n <= bytes && ptr) return CTG_OK;
+    release();
+    if (n == 0) return CTG_OK;
+    hipError_t e = hipMalloc(&ptr, n);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        bytes = 0;
+        (void)hipGetLastError();
+        return fail(CTG_ERROR_OUT_OF_MEMORY, "device allocation of %zu bytes failed: %s", n, hipGetErrorString(e));
+    }
+    bytes = n;
+    return CTG_OK;
+}
+
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+ctg_status fill_u32(void* d, uint32_t value, size_t count, hipStream_t st) {
+    if (count == 0) return CTG_OK;
+    const unsigned blocks = (unsigned)std::min<size_t>((count + 255) / 256, 8192);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks), dim3(256), 0, st, (uint32_t*)d, value, count);
+    CTG_HIP(hipGetLastError());
+    return CTG_OK;
+}
+
+}  // namespace ctg
+
+using namespace ctg;
+
+namespace {
+
+size_t pixel_bytes(ctg_pixel_type p) { return p == CTG_PIXEL_U8 ? 1 : (p == CTG_PIXEL_U16 ? 2 : 4); }
+size_t store_bytes(ctg_store s) { return s == CTG_STORE_F32 ? 8 : 4; }
+
+// Cylinder radius, Ct/CtVolume.cpp:370-373 (unsigned halving, float FCD^2, truncation).
+uint32_t cylinder_radius(uint32_t W, float fcd) {
+    const double r = (double)(W / 2u);
+    const float f2 = fcd * fcd;
+    return (uint32_t)std::sqrt(((double)f2 * r * r) / ((double)f2 + r * r));
+}
+
+// Detector window, Ct/CtVolume.cpp:924-928 with matToImageU/V (:1459-1466).
+void detector_window(ctg_geometry* g) {
+    const uint32_t W = (uint32_t)g->det_width, H = (uint32_t)g->det_height;
+    g->u_pre = (float)((double)(float)W / 2.0);
+    g->v_pre = (float)((double)(float)H / 2.0);
+    g->u_lo = 0.0f - g->u_pre;
+    g->u_hi = (float)((double)(W - 1u) - 0.1) - g->u_pre;
+    g->v_lo = (-1.0f) * (float)((double)(H - 1u) - 0.1) + g->v_pre;
+    g->v_hi = (-1.0f) * 0.0f + g->v_pre;
+}
+
+ctg_status check_ctx(ctg_context* ctx, bool need_geom = true) {
+    if (!ctx) return fail(CTG_ERROR_INVALID_ARGUMENT, "null context");
+    if (need_geom && !ctx->has_geom) return fail(CTG_ERROR_INVALID_ARGUMENT, "geometry not set");
+    CTG_HIP(hipSetDevice(ctx->device));
+    return CTG_OK;
+}
+
+ctg_status check_range(ctg_context* ctx, int p0, int nproj) {
+    if (nproj < 0 || p0 < 0 || p0 + nproj > ctx->nproj_table)
+        return fail(CTG_ERROR_INVALID_ARGUMENT, "projection range [%d,%d) outside the %d set projections", p0,
+                    p0 + nproj, ctx->nproj_table);
+    return CTG_OK;
+}
+
+ctg_status check_slab(ctg_context* ctx, int z0, int nz) {
+    if (z0 < 0 || nz < 0 || z0 + nz > ctx->geom.nz)
+        return fail(CTG_ERROR_INVALID_ARGUMENT, "slab [%d,%d) outside the grid z-range [0,%d)", z0, z0 + nz,
+                    ctx->geom.nz);
+    return CTG_OK;
+}
+
+// Preprocess nproj raw images into the BP layout in batches that bound the FP32 work buffer.
+ctg_status preprocess_batched(ctg_context* ctx, const void* d_raw, ctg_pixel_type pixel, int nproj, float base,
+                              ctg_log_form form, ctg_filter filter, ctg_store store, void* d_filtered,
+                              float* d_scales, hipStream_t st) {
+    const ctg_geometry& g = ctx->geom;
+    const size_t img = (size_t)g.det_width * g.det_height;
+    const size_t budget = (size_t)1 << 30;  // 1 GiB of FP32 work images per batch
+    int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nproj, budget / (img * 4)));
+    CTG_TRY(ctx->rows_f32.reserve(img * 4 * batch));
+    CTG_TRY(ctx->maxabs.reserve(sizeof(unsigned) * std::max(nproj, 1)));
+    CTG_TRY(fill_u32(ctx->maxabs.ptr, 0u, (size_t)nproj, st));
+    for (int b0 = 0; b0 < nproj; b0 += batch) {
+        if (ctx->cancel.load()) return fail(CTG_ERROR_INTERRUPTED, "interrupted");
+        const int nb = std::min(batch, nproj - b0);
+        const char* raw = (const char*)d_raw + (size_t)b0 * img * pixel_bytes(pixel);
+        CTG_TRY(launch_preprocess_rows(ctx, raw, pixel, nb, base, form, filter, ctx->rows_f32.as<float>(),
+                                       ctx->maxabs.as<unsigned>() + b0, true, true, true, st));
+        char* out = (char*)d_filtered + (size_t)b0 * img * store_bytes(store);
+        CTG_TRY(launch_layout(g, ctx->rows_f32.as<float>(), nb, store, ctx->maxabs.as<unsigned>() + b0, out,
+                              d_scales ? d_scales + b0 : nullptr, st));
+    }
+    return CTG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ctg_last_error(void) { return g_last_error.c_str(); }
+
+const char* ctg_version(void) { return "ctg 0.1 (gfx950)"; }
+
+ctg_status ctg_device_count(int32_t* count) {
+    if (!count) return fail(CTG_ERROR_INVALID_ARGUMENT, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *count = 0;
+        return CTG_OK;
+    }
+    *count = n;
+    return CTG_OK;
+}
+
+ctg_status ctg_get_device_info(int32_t device, ctg_device_info* info) {
+    if (!info) return fail(CTG_ERROR_INVALID_ARGUMENT, "null info");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+        (void)hipGetLastError();
+        return fail(CTG_ERROR_NO_DEVICE, "device %d not available", device);
+    }
+    hipDeviceProp_t prop;
+    CTG_HIP(hipGetDeviceProperties(&prop, device));
+    memset(info, 0, sizeof(*info));
+    snprintf(info->name, sizeof(info->name), "%s (%s)", prop.name, prop.gcnArchName);
+    CTG_HIP(hipSetDevice(device));
+    size_t fr = 0, tot = 0;
+    CTG_HIP(hipMemGetInfo(&fr, &tot));
+    info->free_bytes = fr;
+    info->total_bytes = tot;
+    info->compute_units = prop.multiProcessorCount;
+    info->memory_bus_width_bits = prop.memoryBusWidth;
+    info->memory_clock_khz = prop.memoryClockRate;
+    info->clock_khz = prop.clockRate;
+    return CTG_OK;
+}
+
+ctg_status ctg_geometry_ctvolume(uint32_t W, uint32_t H, float fcd_px, float u_offset_px, const float* bounds,
+                                 ctg_geometry* out, uint32_t* roi_out) {
+    clear_error();
+    if (!out) return fail(CTG_ERROR_INVALID_ARGUMENT, "null geometry");
+    if (W < 2 || H < 2) return fail(CTG_ERROR_INVALID_ARGUMENT, "detector %ux%u too small", W, H);
+    // sinogramFromImages rejects imageWidth >= FCD (Ct/CtVolume.cpp:294-301)
+    if (!((float)W < fcd_px)) return fail(CTG_ERROR_INVALID_ARGUMENT, "the FCD must be greater than the image width");
+    const uint32_t R = cylinder_radius(W, fcd_px);
+    const uint32_t xSize = 2u * R, ySize = 2u * R, zSize = H;
+    float xf = 0, xt = 1, yf = 0, yt = 1, zf = 0, zt = 1;
+    if (bounds && !(bounds[0] == bounds[1] || bounds[2] == bounds[3] || bounds[4] == bounds[5])) {
+        // setVolumeBounds, Ct/CtVolume.cpp:493-498 (yTo and zTo are clamped against xFrom there)
+        xf = std::max(0.0f, std::min(1.0f, bounds[0]));
+        xt = std::max(xf, std::min(1.0f, bounds[1]));
+        yf = std::max(0.0f, std::min(1.0f, bounds[2]));
+        yt = std::max(xf, std::min(1.0f, bounds[3]));
+        zf = std::max(0.0f, std::min(1.0f, bounds[4]));
+        zt = std::max(xf, std::min(1.0f, bounds[5]));
+    }
+    // updateBoundaries, Ct/CtVolume.cpp:1401-1423
+    const uint32_t xFrom = (uint32_t)((double)(xf * (float)xSize) + 0.5);
+    const uint32_t xTo = (uint32_t)((double)(xt * (float)xSize) + 0.5);
+    const uint32_t yFrom = (uint32_t)((double)(yf * (float)ySize) + 0.5);
+    const uint32_t yTo = (uint32_t)((double)(yt * (float)ySize) + 0.5);
+    const uint32_t zFrom = (uint32_t)((double)(zf * (float)zSize) + 0.5);
+    const uint32_t zTo = (uint32_t)((double)(zt * (float)zSize) + 0.5);
+    if (roi_out) {
+        roi_out[0] = xFrom; roi_out[1] = xTo; roi_out[2] = yFrom;
+        roi_out[3] = yTo; roi_out[4] = zFrom; roi_out[5] = zTo;
+    }
+    if (xTo < xFrom || yTo < yFrom || zTo < zFrom)
+        return fail(CTG_ERROR_INVALID_ARGUMENT, "volume bounds give a negative extent");
+    memset(out, 0, sizeof(*out));
+    out->det_width = (int32_t)W;
+    out->det_height = (int32_t)H;
+    out->nx = (int32_t)(xTo - xFrom);
+    out->ny = (int32_t)(yTo - yFrom);
+    out->nz = (int32_t)(zTo - zFrom);
+    out->fcd = fcd_px;
+    out->u_offset = u_offset_px;
+    out->radius = (float)R;
+    out->x_pre = (float)(((double)(float)xSize / 2.0) - (double)(float)xFrom);
+    out->y_pre = (float)(((double)(float)ySize / 2.0) - (double)(float)yFrom);
+    out->z_pre = (float)(((double)(float)zSize / 2.0) - (double)(float)zFrom);
+    detector_window(out);
+    return CTG_OK;
+}
+
+ctg_status ctg_geometry_grid(uint32_t W, uint32_t H, float fcd_px, float u_offset_px, int32_t nx, int32_t ny,
+                             int32_t nz, ctg_geometry* out) {
+    clear_error();
+    if (!out) return fail(CTG_ERROR_INVALID_ARGUMENT, "null geometry");
+    if (W < 2 || H < 2 || nx < 0 || ny < 0 || nz < 0) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad sizes");
+    if (!((float)W < fcd_px)) return fail(CTG_ERROR_INVALID_ARGUMENT, "the FCD must be greater than the image width");
+    memset(out, 0, sizeof(*out));
+    out->det_width = (int32_t)W;
+    out->det_height = (int32_t)H;
+    out->nx = nx;
+    out->ny = ny;
+    out->nz = nz;
+    out->fcd = fcd_px;
+    out->u_offset = u_offset_px;
+    out->radius = (float)cylinder_radius(W, fcd_px);
+    out->x_pre = (float)((double)(float)nx / 2.0);
+    out->y_pre = (float)((double)(float)ny / 2.0);
+    out->z_pre = (float)((double)(float)nz / 2.0);
+    detector_window(out);
+    return CTG_OK;
+}
+
+ctg_status ctg_create(int32_t device, ctg_context** out) {
+    clear_error();
+    if (!out) return fail(CTG_ERROR_INVALID_ARGUMENT, "null output");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return fail(CTG_ERROR_NO_DEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= n) return fail(CTG_ERROR_NO_DEVICE, "device %d not available (%d devices)", device, n);
+    CTG_HIP(hipSetDevice(device));
+    ctg_context* c = new (std::nothrow) ctg_context();
+    if (!c) return fail(CTG_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    c->device = device;
+    *out = c;
+    return CTG_OK;
+}
+
+ctg_status ctg_destroy(ctg_context* ctx) {
+    if (!ctx) return CTG_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    ctx->proj_dev.release();
+    ctx->proj_dev64.release();
+    ctx->ftab.resp_f.release(); ctx->ftab.tw_f.release(); ctx->ftab.conv_f.release();
+    ctx->ftab.resp_d.release(); ctx->ftab.tw_d.release(); ctx->ftab.conv_d.release();
+    ctx->rows_f32.release();
+    ctx->rows_f64.release();
+    ctx->maxabs.release();
+    ctx->scales_tmp.release();
+    ctx->filtered.release();
+    ctx->raw_stage.release();
+    ctx->ell.release();
+    delete ctx;
+    return CTG_OK;
+}
+
+ctg_status ctg_set_geometry(ctg_context* ctx, const ctg_geometry* g) {
+    clear_error();
+    CTG_TRY(check_ctx(ctx, false));
+    if (!g) return fail(CTG_ERROR_INVALID_ARGUMENT, "null geometry");
+    if (g->det_width < 2 || g->det_height < 2 || g->nx < 0 || g->ny < 0 || g->nz < 0 || !(g->fcd > 0.0f))
+        return fail(CTG_ERROR_INVALID_ARGUMENT, "invalid geometry");
+    ctx->geom = *g;
+    ctx->has_geom = true;
+    return CTG_OK;
+}
+
+ctg_status ctg_set_projections(ctg_context* ctx, const double* angles_deg, const double* heights_px, int32_t count) {
+    clear_error();
+    CTG_TRY(check_ctx(ctx, false));
+    if (count < 0 || (count > 0 && !angles_deg)) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad projection table");
+    ctx->proj_host.resize(count);
+    ctx->proj_host64.resize(count);
+    for (int i = 0; i < count; ++i) {
+        // Ct/CtVolume.cpp:1200-1202: angle_rad = angle/180*pi, sin/cos in double
+        const double a = (angles_deg[i] / 180.0) * 3.14159265358979323846;
+        const double sn = std::sin(a), cs = std::cos(a);
+        const double h = heights_px ? heights_px[i] : 0.0;
+        ctx->proj_host[i] = make_float4((float)sn, (float)cs, (float)h, 0.0f);
+        ctx->proj_host64[i] = make_double4(sn, cs, h, 0.0);
+    }
+    CTG_TRY(ctx->proj_dev.reserve(sizeof(float4) * std::max(count, 1)));
+    CTG_TRY(ctx->proj_dev64.reserve(sizeof(double4) * std::max(count, 1)));
+    if (count > 0) {
+        CTG_HIP(hipMemcpy(ctx->proj_dev.ptr, ctx->proj_host.data(), sizeof(float4) * count, hipMemcpyHostToDevice));
+        CTG_HIP(hipMemcpy(ctx->proj_dev64.ptr, ctx->proj_host64.data(), sizeof(double4) * count,
+                          hipMemcpyHostToDevice));
+    }
+    ctx->nproj_table = count;
+    return CTG_OK;
+}
+
+void ctg_cancel(ctg_context* ctx) {
+    if (ctx) ctx->cancel.store(true);
+}
+
+void ctg_reset_cancel(ctg_context* ctx) {
+    if (ctx) ctx->cancel.store(false);
+}
+
+ctg_status ctg_set_spare_memory(ctg_context* ctx, uint64_t mib) {
+    if (!ctx) return fail(CTG_ERROR_INVALID_ARGUMENT, "null context");
+    ctx->spare_mib = mib;
+    return CTG_OK;
+}
+
+size_t ctg_filtered_bytes(const ctg_geometry* g, ctg_store store, int32_t nproj) {
+    if (!g || nproj < 0) return 0;
+    return (size_t)g->det_width * g->det_height * store_bytes(store) * (size_t)nproj;
+}
+
+ctg_status ctg_preprocess(ctg_context* ctx, const void* d_raw, ctg_pixel_type pixel, int32_t nproj,
+                          float base_intensity, ctg_log_form log_form, ctg_filter filter, ctg_store store,
+                          void* d_filtered, float* d_scales, void* stream) {
+    clear_error();
+    CTG_TRY(check_ctx(ctx));
+    if (nproj < 0 || (nproj > 0 && (!d_raw || !d_filtered)))
+        return fail(CTG_ERROR_INVALID_ARGUMENT, "bad buffers");
+    if (base_intensity == 0.0f)  // Ct/CtVolume.cpp:177-181
+        return fail(CTG_ERROR_INVALID_ARGUMENT, "the base intensity must not be equal to zero");
+    if ((int)pixel < 0 || (int)pixel > 2 || (int)filter < 0 || (int)filter > 2 || (int)store < 0 || (int)store > 2)
+        return fail(CTG_ERROR_INVALID_ARGUMENT, "bad enum value");
+    if (nproj == 0) return CTG_OK;
+    return preprocess_batched(ctx, d_raw, pixel, nproj, base_intensity, log_form, filter, store, d_filtered,
+                              d_scales, (hipStream_t)stream);
+}
+
+ctg_status ctg_feldkamp_weight(ctg_context* ctx, float* d_images, int32_t nproj, void* stream) {
+    clear_error();
+    CTG_TRY(check_ctx(ctx));
+    if (nproj <= 0) return CTG_OK;
+    return launch_preprocess_rows(ctx, d_images, CTG_PIXEL_F32, nproj, 1.0f, CTG_LOG_CPU_FORM, CTG_FILTER_RAMLAK,
+                                  d_images, nullptr, false, true, false, (hipStream_t)stream);
+}
+
+ctg_status ctg_filter_rows(ctg_context* ctx, const float* d_in, float* d_out, int64_t rows, ctg_filter filter,
+                           void* stream) {
+    clear_error();
+    CTG_TRY(check_ctx(ctx));
+    if (rows <= 0) return CTG_OK;
+    if (!d_in || !d_out) return fail(CTG_ERROR_INVALID_ARGUMENT, "null buffers");
+    return launch_filter_rows_only(ctx, d_in, d_out, rows, filter, (hipStream_t)stream);
+}
+
+ctg_status ctg_layout_projections(ctg_context* ctx, const float* d_images, int32_t nproj, ctg_store store,
+                                  void* d_filtered, float* d_scales, void* stream) {
+    clear_error();
+    CTG_TRY(check_ctx(ctx));
+    if (nproj <= 0) return CTG_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t img = (size_t)ctx->geom.det_width * ctx->geom.det_height;
+    CTG_TRY(ctx->maxabs.reserve(sizeof(unsigned) * nproj));
+    CTG_TRY(fill_u32(ctx->maxabs.ptr, 0u, (size_t)nproj, st));
+    if (store != CTG_STORE_F32) {
+        const unsigned bx = (unsigned)std::min<size_t>((img + 255) / 256, 64);
+        hipLaunchKernelGGL(maxabs_kernel, dim3(bx, (unsigned)nproj), dim3(256), 0, st, d_images, img,
+                           ctx->maxabs.as<unsigned>());
+        CTG_HIP(hipGetLastError());
+    }
+    return launch_layout(ctx->geom, d_images, nproj, store, ctx->maxabs.as<unsigned>(), d_filtered, d_scales, st);
+}
+
+ctg_status ctg_backproject(ctg_context* ctx, const void* d_filtered, ctg_store store, const float* d_scales,
+                           int32_t p0, int32_t nproj, float* d_volume, int32_t z0, int32_t nz, int32_t accumulate,
+                           void* stream) {
+    clear_error();
+    CTG_TRY(check_ctx(ctx));
+    CTG_TRY(check_range(ctx, p0, nproj));
+    CTG_TRY(check_slab(ctx, z0, nz));
+    if ((int)store < 0 || (int)store > 2) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad store");
+    if (!d_volume || (nproj > 0 && !d_filtered)) return fail(CTG_ERROR_INVALID_ARGUMENT, "null buffers");
+    if (ctx->cancel.load()) return fail(CTG_ERROR_INTERRUPTED, "interrupted");
+    BpLaunch L{};
+    L.g = &ctx->geom;
+    L.d_params = ctx->proj_dev.as<float4>() + p0;
+    L.d_scales = d_scales;
+    L.d_filtered = d_filtered;
+    L.store = store;
+    L.nproj = nproj;
+    L.d_volume = d_volume;
+    L.z0 = z0;
+    L.nz = nz;
+    L.accumulate = accumulate ? 1 : 0;
+    return launch_backproject(L, (hipStream_t)stream);
+}
+
+ctg_status ctg_forward_project(ctg_context* ctx, const float* d_volume, int32_t z0, int32_t nz, int32_t p0,
+                               int32_t nproj, float* d_proj, int32_t accumulate, void* stream) {
+    clear_error();
+    CTG_TRY(check_ctx(ctx));
+    CTG_TRY(check_range(ctx, p0, nproj));
+    CTG_TRY(check_slab(ctx, z0, nz));
+    if (!d_volume || (nproj > 0 && !d_proj)) return fail(CTG_ERROR_INVALID_ARGUMENT, "null buffers");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t n = (size_t)nproj * ctx->geom.det_width * ctx->geom.det_height;
+    if (!accumulate) CTG_TRY(fill_u32(d_proj, 0u, n, st));
+    return launch_forward_project(ctx->geom, ctx->proj_dev.as<float4>() + p0, d_volume, z0, nz, nproj, d_proj, st);
+}
+
+ctg_status ctg_fdk_backward(ctg_context* ctx, const float* d_grad_volume, int32_t z0, int32_t nz, int32_t p0,
+                            int32_t nproj, ctg_filter filter, float* d_grad_lineint, void* stream) {
+    clear_error();
+    CTG_TRY(check_ctx(ctx));
+    CTG_TRY(check_range(ctx, p0, nproj));
+    CTG_TRY(check_slab(ctx, z0, nz));
+    if (nproj == 0) return CTG_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const ctg_geometry& g = ctx->geom;
+    const size_t n = (size_t)nproj * g.det_width * g.det_height;
+    CTG_TRY(fill_u32(d_grad_lineint, 0u, n, st));
+    CTG_TRY(launch_forward_project(g, ctx->proj_dev.as<float4>() + p0, d_grad_volume, z0, nz, nproj, d_grad_lineint,
+                                   st));
+    CTG_TRY(launch_filter_rows_only(ctx, d_grad_lineint, d_grad_lineint, (int64_t)nproj * g.det_height, filter, st));
+    return launch_preprocess_rows(ctx, d_grad_lineint, CTG_PIXEL_F32, nproj, 1.0f, CTG_LOG_CPU_FORM, filter,
+                                  d_grad_lineint, nullptr, false, true, false, st);
+}
+
+ctg_status ctg_fdk_f64(ctg_context* ctx, const void* d_raw, ctg_pixel_type pixel, int32_t nproj, float base_intensity,
+                       ctg_filter filter, double* d_volume, int32_t z0, int32_t nz, void* stream) {
+    clear_error();
+    CTG_TRY(check_ctx(ctx));
+    CTG_TRY(check_range(ctx, 0, nproj));
+    CTG_TRY(check_slab(ctx, z0, nz));
+    if (base_intensity == 0.0f) return fail(CTG_ERROR_INVALID_ARGUMENT, "the base intensity must not be equal to zero");
+    hipStream_t st = (hipStream_t)stream;
+    const ctg_geometry& g = ctx->geom;
+    const size_t img = (size_t)g.det_width * g.det_height;
+    CTG_TRY(ctx->rows_f64.reserve(img * 8 * std::max(nproj, 1)));
+    CTG_TRY(launch_preprocess_rows_f64(ctx, d_raw, pixel, nproj, base_intensity, filter, ctx->rows_f64.as<double>(),
+                                       st));
+    return launch_backproject_f64(g, ctx->proj_dev64.as<double4>(), ctx->rows_f64.as<double>(), nproj, d_volume, z0,
+                                  nz, st);
+}
+
+ctg_status ctg_convert_volume(ctg_context* ctx, const float* d_volume, int32_t nz, ctg_index_order order,
+                              ctg_byte_order byte_order, void* d_out, void* stream) {
+    clear_error();
+    CTG_TRY(check_ctx(ctx));
+    if (!d_volume || !d_out || nz < 0) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad buffers");
+    return launch_convert_volume(ctx->geom, d_volume, nz, order, byte_order, d_out, (hipStream_t)stream);
+}
+
+ctg_status ctg_phantom_images(ctg_context* ctx, const double* ell, int32_t n_ell, int32_t p0, int32_t nproj,
+                              double photons, double full_scale, uint64_t seed, uint16_t* d_images, float* d_lineint,
+                              void* stream) {
+    clear_error();
+    CTG_TRY(check_ctx(ctx));
+    CTG_TRY(check_range(ctx, p0, nproj));
+    if (n_ell < 0 || (n_ell > 0 && !ell)) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad ellipsoid list");
+    if (nproj == 0) return CTG_OK;
+    return launch_phantom(ctx->geom, ctx->proj_dev64.as<double4>(), ell, n_ell, nproj, p0, photons, full_scale, seed,
+                          d_images, d_lineint, ctx->ell, (hipStream_t)stream);
+}
+
+// Slab pipeline.  Raw images go host -> pinned staging -> device in batches on an upload
+// stream while the compute stream preprocesses the previous batch into the resident
+// backprojection layout; then every z-slab is backprojected from all projections,
+// converted to the requested index order and copied out.  When the filtered projections
+// do not fit next to a useful slab, they are recomputed per slab from re-uploaded images.
+ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type pixel, int32_t nproj,
+                           float base_intensity, ctg_log_form log_form, ctg_filter filter, ctg_store store,
+                           float* h_volume, ctg_index_order order, ctg_progress_fn progress, void* user) {
+    clear_error();
+    CTG_TRY(check_ctx(ctx));
+    CTG_TRY(check_range(ctx, 0, nproj));
+    if (!h_raw || !h_volume || nproj <= 0) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad buffers");
+    if (base_intensity == 0.0f) return fail(CTG_ERROR_INVALID_ARGUMENT, "the base intensity must not be equal to zero");
+    const ctg_geometry& g = ctx->geom;
+    const size_t img = (size_t)g.det_width * g.det_height;
+    const size_t pb = pixel_bytes(pixel);
+    const size_t col = (size_t)g.nx * g.ny;
+    if (col == 0 || g.nz == 0) return CTG_OK;
+
+    size_t fr = 0, tot = 0;
+    CTG_HIP(hipMemGetInfo(&fr, &tot));
+    const long long spare = (long long)ctx->spare_mib * 1024ll * 1024ll;
+    long long avail = (long long)fr - spare;
+    // buffers owned by the context count as available (they are reused)
+    avail += (long long)ctx->filtered.bytes + (long long)ctx->raw_stage.bytes + (long long)ctx->rows_f32.bytes;
+    const int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nproj, ((size_t)256 << 20) / (img * 4)));
+    const long long work = (long long)img * batch * (4 + 2 * pb) + (long long)nproj * 8;
+    const long long filt = (long long)ctg_filtered_bytes(&g, store, nproj);
+    const long long slab_col = (long long)col * 4 * 2;  // slab + converted copy, per z
+    bool resident = avail - work - filt >= slab_col * std::min(g.nz, 32);
+    long long for_slabs = avail - work - (resident ? filt : (long long)ctg_filtered_bytes(&g, store, batch));
+    if (for_slabs < slab_col) {
+        return fail(CTG_ERROR_INSUFFICIENT_VRAM,
+                    "The VRAM of the GPU is insufficient to run a reconstruction (free %zu MiB, spare %llu MiB).",
+                    fr >> 20, (unsigned long long)ctx->spare_mib);
+    }
+    int slab_nz = (int)std::min<long long>(g.nz, for_slabs / slab_col);
+    if (slab_nz >= 32) slab_nz -= slab_nz % 32;
+
+    hipStream_t s_up = nullptr, s_cmp = nullptr;
+    hipEvent_t ev_up[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr};
+    void* pinned[2] = {nullptr, nullptr};
+    float* d_slab = nullptr;
+    void* d_conv = nullptr;
+    ctg_status st = CTG_OK;
+    auto cleanup = [&]() {
+        if (s_up) (void)hipStreamSynchronize(s_up);
+        if (s_cmp) (void)hipStreamSynchronize(s_cmp);
+        for (int i = 0; i < 2; ++i) {
+            if (ev_up[i]) (void)hipEventDestroy(ev_up[i]);
+            if (ev_done[i]) (void)hipEventDestroy(ev_done[i]);
+            if (pinned[i]) (void)hipHostFree(pinned[i]);
+        }
+        if (s_up) (void)hipStreamDestroy(s_up);
+        if (s_cmp) (void)hipStreamDestroy(s_cmp);
+        if (d_slab) (void)hipFree(d_slab);
+        if (d_conv) (void)hipFree(d_conv);
+    };
+#define PIPE_HIP(call)                                                                                      \
+    do {                                                                                                    \
+        hipError_t e_ = (call);                                                                             \
+        if (e_ != hipSuccess) {                                                                             \
+            st = fail(e_ == hipErrorOutOfMemory ? CTG_ERROR_OUT_OF_MEMORY : CTG_ERROR_DEVICE, "%s failed: %s", \
+                      #call, hipGetErrorString(e_));                                                        \
+            cleanup();                                                                                      \
+            return st;                                                                                      \
+        }                                                                                                   \
+    } while (0)
+#define PIPE_TRY(expr)            \
+    do {                          \
+        st = (expr);              \
+        if (st != CTG_OK) {       \
+            cleanup();            \
+            return st;            \
+        }                         \
+    } while (0)
+
+    PIPE_HIP(hipStreamCreateWithFlags(&s_up, hipStreamNonBlocking));
+    PIPE_HIP(hipStreamCreateWithFlags(&s_cmp, hipStreamNonBlocking));
+    for (int i = 0; i < 2; ++i) {
+        PIPE_HIP(hipEventCreateWithFlags(&ev_up[i], hipEventDisableTiming));
+        PIPE_HIP(hipEventCreateWithFlags(&ev_done[i], hipEventDisableTiming));
+        PIPE_HIP(hipHostMalloc(&pinned[i], img * pb * batch, hipHostMallocDefault));
+    }
+    PIPE_TRY(ctx->raw_stage.reserve(img * pb * batch * 2));
+    PIPE_TRY(ctx->filtered.reserve((size_t)(resident ? filt : (long long)ctg_filtered_bytes(&g, store, nproj))));
+    PIPE_TRY(ctx->scales_tmp.reserve(sizeof(float) * nproj));
+    PIPE_HIP(hipMalloc((void**)&d_slab, col * 4 * (size_t)slab_nz));
+    if (order == CTG_X_FASTEST) PIPE_HIP(hipMalloc(&d_conv, col * 4 * (size_t)slab_nz));
+
+    // Upload + preprocess all projections (double-buffered), into ctx->filtered.
+    auto preprocess_all = [&]() -> ctg_status {
+        const int nb_total = (nproj + batch - 1) / batch;
+        for (int b = 0; b < nb_total; ++b) {
+            if (ctx->cancel.load()) return fail(CTG_ERROR_INTERRUPTED, "interrupted");
+            const int i = b & 1;
+            const int b0 = b * batch, nb = std::min(batch, nproj - b0);
+            // staging buffer i is free once the preprocessing that read it has finished
+            CTG_HIP(hipEventSynchronize(ev_done[i]));
+            memcpy(pinned[i], (const char*)h_raw + (size_t)b0 * img * pb, (size_t)nb * img * pb);
+            char* dst = (char*)ctx->raw_stage.ptr + (size_t)i * img * pb * batch;
+            CTG_HIP(hipMemcpyAsync(dst, pinned[i], (size_t)nb * img * pb, hipMemcpyHostToDevice, s_up));
+            CTG_HIP(hipEventRecord(ev_up[i], s_up));
+            CTG_HIP(hipStreamWaitEvent(s_cmp, ev_up[i], 0));
+            char* out = (char*)ctx->filtered.ptr + (size_t)b0 * img * store_bytes(store);
+            CTG_TRY(preprocess_batched(ctx, dst, pixel, nb, base_intensity, log_form, filter, store, out,
+                                       ctx->scales_tmp.as<float>() + b0, s_cmp));
+            CTG_HIP(hipEventRecord(ev_done[i], s_cmp));
+        }
+        return CTG_OK;
+    };
+
+    if (resident) PIPE_TRY(preprocess_all());
+    const int nslabs = (g.nz + slab_nz - 1) / slab_nz;
+    for (int s = 0; s < nslabs; ++s) {
+        if (ctx->cancel.load()) {
+            st = fail(CTG_ERROR_INTERRUPTED, "interrupted");
+            cleanup();
+            return st;
+        }
+        const int z0 = s * slab_nz, nz = std::min(slab_nz, g.nz - z0);
+        if (!resident) PIPE_TRY(preprocess_all());
+        BpLaunch L{};
+        L.g = &g;
+        L.d_params = ctx->proj_dev.as<float4>();
+        L.d_scales = ctx->scales_tmp.as<float>();
+        L.d_filtered = ctx->filtered.ptr;
+        L.store = store;
+        L.nproj = nproj;
+        L.d_volume = d_slab;
+        L.z0 = z0;
+        L.nz = nz;
+        L.accumulate = 0;
+        PIPE_TRY(launch_backproject(L, s_cmp));
+        if (order == CTG_X_FASTEST) {
+            PIPE_TRY(launch_convert_volume(g, d_slab, nz, CTG_X_FASTEST, CTG_LITTLE_ENDIAN, d_conv, s_cmp));
+            PIPE_HIP(hipMemcpyAsync(h_volume + (size_t)z0 * col, d_conv, col * 4 * (size_t)nz, hipMemcpyDeviceToHost,
+                                    s_cmp));
+        } else {
+            // [x][y][z0:z0+nz] of every column: a strided copy into the Z_FASTEST host volume
+            PIPE_HIP(hipMemcpy2DAsync(h_volume + z0, (size_t)g.nz * 4, d_slab, (size_t)nz * 4, (size_t)nz * 4, col,
+                                      hipMemcpyDeviceToHost, s_cmp));
+        }
+        PIPE_HIP(hipStreamSynchronize(s_cmp));
+        if (progress) progress((double)(s + 1) / nslabs, user);
+    }
+    cleanup();
+#undef PIPE_HIP
+#undef PIPE_TRY
+    return CTG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,284 @@
+"""GPU parity: libctg (HIP, through the C ABI) against the CPU oracle on the same inputs.
+
+Bars: the plain FP32 path matches the faithful CPU restatement of
+CtVolume::reconstructionCore + CPU preprocessing within 1e-4 relative RMSE
+(Ct/CtVolume.cpp:787-885, 923-1054); FP64 GPU path matches the FP64 oracle within
+1e-10; FP16-stored projections within 1e-3 of the FP64 result.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import pairs_to_rows, rel_rmse, small_scan
+
+pytestmark = pytest.mark.gpu
+
+FDK_TOL = 1e-4
+
+
+def _ctx(scan, bounds=None, grid=None):
+    import ct_amd
+    if grid is not None:
+        g = ct_amd.geometry_grid(scan["W"], scan["H"], scan["fcd"], scan["u_offset"], *grid)
+    else:
+        g, _ = ct_amd.geometry_ctvolume(scan["W"], scan["H"], scan["fcd"], scan["u_offset"], bounds)
+    ctx = ct_amd.Context(0)
+    ctx.set_geometry(g)
+    ctx.set_projections(scan["angles"], scan["heights"])
+    return ctx, g
+
+
+def _gpu_fdk(ctx, g, scan, store="f32", filt="ramlak", log_form="cpu", z0=0, nz=None):
+    import torch
+    nz = g.nz if nz is None else nz
+    P = scan["P"]
+    raw = torch.from_numpy(scan["raw"]).cuda()
+    filtered = torch.empty(ctx.filtered_bytes(store, P), dtype=torch.uint8, device="cuda")
+    scales = torch.empty(P, dtype=torch.float32, device="cuda")
+    vol = torch.empty((g.nx, g.ny, nz), dtype=torch.float32, device="cuda")
+    ctx.preprocess(raw, "u16", P, scan["base"], log_form, filt, store, filtered, scales)
+    ctx.backproject(filtered, store, scales, 0, P, vol, z0, nz)
+    torch.cuda.synchronize()
+    return vol.cpu().numpy(), filtered, scales
+
+
+def _oracle_geom(g):
+    import oracle as orc
+    return orc.geometry_from(g)
+
+
+def test_preprocess_matches_oracle(gpu, oracle):
+    import torch
+    scan = small_scan()
+    ctx, g = _ctx(scan)
+    P, W, H = scan["P"], scan["W"], scan["H"]
+    for filt in ("ramlak", "shepplogan", "hann"):
+        for log_form in ("cpu", "gpu"):
+            raw = torch.from_numpy(scan["raw"]).cuda()
+            filtered = torch.empty((P, W, H, 2), dtype=torch.float32, device="cuda")
+            scales = torch.empty(P, dtype=torch.float32, device="cuda")
+            ctx.preprocess(raw, "u16", P, scan["base"], log_form, filt, "f32", filtered, scales)
+            torch.cuda.synchronize()
+            got = pairs_to_rows(filtered.cpu().numpy(), W, H)
+            ref = oracle.preprocess_f32(_oracle_geom(g), scan["raw"], "u16", scan["base"], filt, log_form)
+            err = rel_rmse(got, ref)
+            print(f"preprocess {filt}/{log_form}: rel RMSE {err:.3e}")
+            assert err < 2e-6, (filt, log_form, err)
+            # second tap of each pair is the right-hand neighbour, 0 past the last column
+            pr = filtered.cpu().numpy()
+            assert np.array_equal(pr[:, :-1, :, 1], pr[:, 1:, :, 0])
+            assert np.all(pr[:, -1, :, 1] == 0)
+            assert np.all(scales.cpu().numpy() == 1.0)
+
+
+def test_backproject_matches_oracle(gpu, oracle):
+    import torch
+    scan = small_scan(P=30)
+    ctx, g = _ctx(scan)
+    og = _oracle_geom(g)
+    q = oracle.preprocess_f32(og, scan["raw"], "u16", scan["base"], "ramlak")
+    ref = oracle.backproject_f32(og, q, scan["angles"], scan["heights"])
+    P = scan["P"]
+    qd = torch.from_numpy(q).cuda()
+    filtered = torch.empty(ctx.filtered_bytes("f32", P), dtype=torch.uint8, device="cuda")
+    ctx.layout_projections(qd, P, "f32", filtered, None)
+    vol = torch.full((g.nx, g.ny, g.nz), 7.0, dtype=torch.float32, device="cuda")
+    ctx.backproject(filtered, "f32", None, 0, P, vol, 0, g.nz)
+    torch.cuda.synchronize()
+    got = vol.cpu().numpy()
+    err = rel_rmse(got, ref)
+    print(f"backprojection rel RMSE {err:.3e}")
+    assert err < 1e-5
+    # voxels outside the reconstruction cylinder are exactly zero, as in the reference
+    xs = np.arange(g.nx) - g.x_pre
+    ys = np.arange(g.ny) - g.y_pre
+    outside = (xs[:, None] ** 2 + ys[None, :] ** 2) >= g.radius ** 2
+    assert np.all(got[outside] == 0.0)
+
+
+@pytest.mark.parametrize("filt", ["ramlak", "shepplogan", "hann"])
+def test_fdk_parity_ctvolume_grid(gpu, oracle, filt):
+    scan = small_scan(W=96, H=64, P=40, seed=3)
+    ctx, g = _ctx(scan)
+    got, _, _ = _gpu_fdk(ctx, g, scan, filt=filt)
+    ref = oracle.fdk_f32(_oracle_geom(g), scan["raw"], "u16", scan["base"], filt, scan["angles"], scan["heights"])
+    err = rel_rmse(got, ref)
+    print(f"FDK {filt} {g.nx}x{g.ny}x{g.nz}: rel RMSE {err:.3e}")
+    assert err < FDK_TOL
+
+
+def test_fdk_parity_roi_and_offsets(gpu, oracle):
+    """ROI bounds, non-zero axis offset and height offsets, clockwise (negative) angles."""
+    P = 32
+    heights = np.linspace(-1.5, 1.5, P)
+    angles = -np.arange(P) * 360.0 / P
+    scan = small_scan(W=80, H=56, P=P, seed=5, u_offset=1.25, heights=heights, angles=angles)
+    bounds = [0.2, 0.8, 0.25, 0.7, 0.1, 0.9]
+    ctx, g = _ctx(scan, bounds=bounds)
+    assert (g.nx, g.ny) != (g.nx + 0, 0)
+    got, _, _ = _gpu_fdk(ctx, g, scan, filt="shepplogan")
+    ref = oracle.fdk_f32(_oracle_geom(g), scan["raw"], "u16", scan["base"], "shepplogan", scan["angles"],
+                         scan["heights"])
+    err = rel_rmse(got, ref)
+    print(f"ROI FDK {g.nx}x{g.ny}x{g.nz}: rel RMSE {err:.3e}")
+    assert err < FDK_TOL
+
+
+def test_fdk_parity_odd_sizes_and_cubic_grid(gpu, oracle):
+    """Non power-of-two detector width (circular-convolution filter path), odd height,
+    explicit grid whose nz is not a multiple of the z-run."""
+    scan = small_scan(W=90, H=45, P=20, seed=9)
+    ctx, g = _ctx(scan, grid=(70, 66, 45))
+    got, _, _ = _gpu_fdk(ctx, g, scan, filt="hann")
+    ref = oracle.fdk_f32(_oracle_geom(g), scan["raw"], "u16", scan["base"], "hann", scan["angles"], scan["heights"])
+    err = rel_rmse(got, ref)
+    print(f"odd-size FDK: rel RMSE {err:.3e}")
+    assert err < FDK_TOL
+
+
+def test_wide_cone_uses_global_path(gpu, oracle):
+    """FCD barely above the width: footprints exceed the LDS tile, the kernel reads global memory."""
+    scan = small_scan(W=64, H=64, P=16, fcd=66.0, seed=21)
+    ctx, g = _ctx(scan)
+    got, _, _ = _gpu_fdk(ctx, g, scan)
+    ref = oracle.fdk_f32(_oracle_geom(g), scan["raw"], "u16", scan["base"], "ramlak", scan["angles"],
+                         scan["heights"])
+    err = rel_rmse(got, ref)
+    print(f"wide-cone FDK: rel RMSE {err:.3e}")
+    assert err < FDK_TOL
+
+
+def test_slabs_and_projection_batches_compose(gpu):
+    import torch
+    scan = small_scan(W=64, H=64, P=24, seed=2)
+    ctx, g = _ctx(scan)
+    full, filtered, scales = _gpu_fdk(ctx, g, scan)
+    P = scan["P"]
+    parts = torch.zeros((g.nx, g.ny, g.nz), dtype=torch.float32, device="cuda")
+    half = P // 2
+    per = ctx.filtered_bytes("f32", 1)
+    # z-slabs written separately, projections split into two accumulating launches
+    for z0, nz in ((0, 20), (20, g.nz - 20)):
+        slab = torch.empty((g.nx, g.ny, nz), dtype=torch.float32, device="cuda")
+        ctx.backproject(filtered, "f32", scales, 0, half, slab, z0, nz)
+        ctx.backproject(filtered[half * per:], "f32", scales[half:], half, P - half, slab, z0, nz, accumulate=True)
+        parts[:, :, z0:z0 + nz] = slab
+    torch.cuda.synchronize()
+    err = rel_rmse(parts.cpu().numpy(), full)
+    assert err < 1e-6
+
+
+def test_fp64_gpu_matches_fp64_oracle(gpu, oracle):
+    import torch
+    scan = small_scan(W=64, H=48, P=20, seed=4)
+    ctx, g = _ctx(scan)
+    vol = torch.empty((g.nx, g.ny, g.nz), dtype=torch.float64, device="cuda")
+    ctx.fdk_f64(torch.from_numpy(scan["raw"]).cuda(), "u16", scan["P"], scan["base"], "shepplogan", vol, 0, g.nz)
+    torch.cuda.synchronize()
+    ref = oracle.fdk_f64(_oracle_geom(g), scan["raw"], "u16", scan["base"], "shepplogan", scan["angles"],
+                         scan["heights"])
+    err = rel_rmse(vol.cpu().numpy(), ref)
+    print(f"FP64 GPU vs FP64 oracle: rel RMSE {err:.3e}")
+    assert err < 1e-10
+
+
+@pytest.mark.parametrize("store,tol", [("f16", 1e-3), ("bf16", 4e-3)])
+def test_reduced_precision_store_vs_fp64(gpu, store, tol):
+    import torch
+    scan = small_scan(W=96, H=64, P=60, seed=8)
+    ctx, g = _ctx(scan)
+    ref = torch.empty((g.nx, g.ny, g.nz), dtype=torch.float64, device="cuda")
+    ctx.fdk_f64(torch.from_numpy(scan["raw"]).cuda(), "u16", scan["P"], scan["base"], "hann", ref, 0, g.nz)
+    got, _, scales = _gpu_fdk(ctx, g, scan, store=store, filt="hann")
+    f32, _, _ = _gpu_fdk(ctx, g, scan, store="f32", filt="hann")
+    refn = ref.cpu().numpy()
+    err = rel_rmse(got, refn)
+    err32 = rel_rmse(f32, refn)
+    sc = scales.cpu().numpy()
+    print(f"{store}: rel RMSE vs FP64 {err:.3e} (FP32 store {err32:.3e}); scales {sc.min():g}..{sc.max():g}")
+    assert np.all(np.log2(sc) == np.round(np.log2(sc)))  # power-of-two scales
+    assert err < tol
+
+
+def test_forward_projector_is_adjoint_of_backprojection(gpu):
+    import torch
+    scan = small_scan(W=64, H=48, P=18, seed=6)
+    ctx, g = _ctx(scan)
+    P, W, H = scan["P"], scan["W"], scan["H"]
+    gen = torch.Generator(device="cpu").manual_seed(4)
+    x = torch.randn((g.nx, g.ny, g.nz), generator=gen, dtype=torch.float32).cuda()
+    y = torch.randn((P, H, W), generator=gen, dtype=torch.float32).cuda()
+    Ax = torch.empty((P, H, W), dtype=torch.float32, device="cuda")
+    ctx.forward_project(x, 0, g.nz, 0, P, Ax)
+    yl = torch.empty(ctx.filtered_bytes("f32", P), dtype=torch.uint8, device="cuda")
+    ctx.layout_projections(y, P, "f32", yl, None)
+    Aty = torch.empty_like(x)
+    ctx.backproject(yl, "f32", None, 0, P, Aty, 0, g.nz)
+    torch.cuda.synchronize()
+    lhs = float(torch.sum(Ax.double() * y.double()))
+    rhs = float(torch.sum(x.double() * Aty.double()))
+    rel = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    print(f"<Ax,y>={lhs:.9e} <x,A^T y>={rhs:.9e} rel {rel:.2e}")
+    assert rel < 1e-5
+
+
+def test_forward_projector_matches_fp64_oracle(gpu, oracle):
+    import torch
+    scan = small_scan(W=64, H=48, P=12, seed=7)
+    ctx, g = _ctx(scan)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((g.nx, g.ny, g.nz)).astype(np.float32)
+    Ax = torch.empty((scan["P"], scan["H"], scan["W"]), dtype=torch.float32, device="cuda")
+    ctx.forward_project(torch.from_numpy(x).cuda(), 0, g.nz, 0, scan["P"], Ax)
+    torch.cuda.synchronize()
+    ref = oracle.forward_project_f64(_oracle_geom(g), x.astype(np.float64), scan["angles"], scan["heights"])
+    err = rel_rmse(Ax.cpu().numpy(), ref)
+    print(f"forward projector vs FP64 oracle: {err:.3e}")
+    assert err < 1e-5
+
+
+def test_phantom_generator_matches_oracle(gpu, oracle):
+    import torch
+    scan = small_scan(W=64, H=48, P=6, seed=12)
+    ctx, g = _ctx(scan)
+    lin = torch.empty((scan["P"], scan["H"], scan["W"]), dtype=torch.float32, device="cuda")
+    img = torch.empty((scan["P"], scan["H"], scan["W"]), dtype=torch.uint16, device="cuda")
+    ctx.phantom_images(scan["ell"], 0, scan["P"], 0.0, 65535.0, 1, img, lin)
+    torch.cuda.synchronize()
+    err = rel_rmse(lin.cpu().numpy(), scan["lineint"])
+    assert err < 1e-6
+    diff = np.abs(img.cpu().numpy().astype(np.int64) - scan["raw"].astype(np.int64))
+    assert diff.max() <= 1
+
+
+def test_reconstruct_pipeline_index_orders(gpu):
+    import torch
+    scan = small_scan(W=64, H=48, P=20, seed=13)
+    ctx, g = _ctx(scan)
+    ref, _, _ = _gpu_fdk(ctx, g, scan, filt="shepplogan")
+    for order in ("zfastest", "xfastest"):
+        out = np.empty(g.nx * g.ny * g.nz, np.float32)
+        raw = np.ascontiguousarray(scan["raw"])
+        ctx.reconstruct(raw.ctypes.data, "u16", scan["P"], scan["base"], "cpu", "shepplogan", "f32",
+                        out.ctypes.data, order)
+        if order == "zfastest":
+            got = out.reshape(g.nx, g.ny, g.nz)
+        else:
+            got = np.transpose(out.reshape(g.nz, g.ny, g.nx), (2, 1, 0))
+        assert np.array_equal(got, ref), order
+
+
+def test_convert_volume_orders_and_byte_swap(gpu):
+    import torch
+    scan = small_scan(W=64, H=48, P=2, seed=1)
+    ctx, g = _ctx(scan)
+    vol = torch.randn((g.nx, g.ny, g.nz), dtype=torch.float32, device="cuda")
+    out = torch.empty(g.nx * g.ny * g.nz, dtype=torch.float32, device="cuda")
+    ctx.convert_volume(vol, g.nz, "xfastest", "littleendian", out)
+    torch.cuda.synchronize()
+    v = vol.cpu().numpy()
+    assert np.array_equal(out.cpu().numpy(), np.transpose(v, (2, 1, 0)).ravel())
+    ctx.convert_volume(vol, g.nz, "zfastest", "bigendian", out)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy().view(">f4").astype(np.float32), v.ravel())
