@@ -96,6 +96,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--only", choices=["f32", "f16", "both"], default="both")
+    ap.add_argument("--calib-copy-gib", type=float, default=0.0,
+                    help="run a device-to-device copy of this size first (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -110,6 +112,15 @@ def main():
     torch.cuda.set_device(dev)
     info = ct_amd._ctg.device_info(0)
     log(f"device: {info.name.decode()} CUs={info.compute_units} HBM={info.total_bytes / 2**30:.0f} GiB")
+    if args.calib_copy_gib > 0:
+        n = int(args.calib_copy_gib * 2**30) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {n * 4} bytes read and written, twice")
+        del src, dst
     ctx = ct_amd.Context(0)
     g, raw, angles, base = build_workload(ctx, torch, args)
     P = args.nproj
@@ -151,6 +162,10 @@ def main():
         return t, bp, filtered, scales
 
     res = {}
+    if args.only == "f16":
+        t16, bp16, _, _ = run_store("f16")
+        print(json.dumps({"bp_ms_f16": bp16 * 1e3, "bp_gups_f16": updates / bp16 / 1e9}), flush=True)
+        return
     t32, bp32, filt32, sc32 = run_store("f32")
     res["f32"] = (t32, bp32)
     vol32 = vol.clone() if not args.no_precision else None
