@@ -172,7 +172,7 @@ def main():
     rows_for_cpu = None
     if not args.no_cpu_baseline:
         # filtered rows (first tap of each pair) for the CPU oracle
-        rows_for_cpu = filt32.view(torch.float32).view(P, g.det_width, g.det_height, 2)[..., 0] \
+        rows_for_cpu = filt32.view(torch.float32).view(P, g.det_width, g.det_height, 4)[..., 0] \
             .transpose(1, 2).contiguous().cpu().numpy()
     del filt32, sc32
     if args.only in ("f16", "both"):

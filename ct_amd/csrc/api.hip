@@ -79,7 +79,7 @@ using namespace ctg;
 namespace {
 
 size_t pixel_bytes(ctg_pixel_type p) { return p == CTG_PIXEL_U8 ? 1 : (p == CTG_PIXEL_U16 ? 2 : 4); }
-size_t store_bytes(ctg_store s) { return s == CTG_STORE_F32 ? 8 : 4; }
+size_t store_bytes(ctg_store s) { return s == CTG_STORE_F32 ? 16 : 8; }
 
 // Cylinder radius, Ct/CtVolume.cpp:370-373 (unsigned halving, float FCD^2, truncation).
 uint32_t cylinder_radius(uint32_t W, float fcd) {
@@ -414,6 +414,7 @@ ctg_status ctg_backproject(ctg_context* ctx, const void* d_filtered, ctg_store s
     BpLaunch L{};
     L.g = &ctx->geom;
     L.d_params = ctx->proj_dev.as<float4>() + p0;
+    L.h_params = ctx->proj_host.data() + p0;
     L.d_scales = d_scales;
     L.d_filtered = d_filtered;
     L.store = store;
@@ -618,6 +619,7 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
         BpLaunch L{};
         L.g = &g;
         L.d_params = ctx->proj_dev.as<float4>();
+        L.h_params = ctx->proj_host.data();
         L.d_scales = ctx->scales_tmp.as<float>();
         L.d_filtered = ctx->filtered.ptr;
         L.store = store;

@@ -10,17 +10,18 @@
 //  - one launch covers many projections; every lane owns one (x,y) voxel column and a
 //    run of ZPL consecutive z, accumulating in registers across all projections, so the
 //    volume slab is written once per launch (z-fastest, each lane stores ZPL*4 bytes);
+//  - projections arrive as bilinear-coefficient entries (A, dU, dV, dUV) stored
+//    column-major, so one voxel-update is one LDS read of one entry and
+//    val = A + fu*dU + fv*(dV + fu*dUV): 3 FMAs, plus 1 FMA into the accumulator;
 //  - a workgroup (256 lanes = 16x16 columns) stages, per projection, the bounding box of
-//    its voxel block's detector footprint into LDS (register prefetch of projection p+1
-//    overlaps the compute of p).  The projection is stored as horizontal tap pairs
-//    (q[v][u], q[v][u+1]) with v fastest, so one lane's four taps are two adjacent LDS
-//    entries (one ds_read2);
-//  - per z the lane does one fma for vIndex, floor/fract, one address add, the two-entry
-//    LDS read, and 7 VALU for the interpolation and accumulation; the per-column geometry
-//    (a division) is amortised over the z-run;
+//    its voxel block's detector footprint into LDS with global->LDS DMA
+//    (global_load_lds), one detector column per DMA burst, double-buffered so that the
+//    DMA of projection p+1 runs under the compute of p;
+//  - per z the lane does one FMA for vIndex, v_cvt_flr/v_fract, one address op, the LDS
+//    read and 4 FMAs; the per-column geometry (one division) is amortised over the run;
 //  - boxes are computed by wave 0 for 64 projections at a time (one lane per projection);
-//    a projection whose box exceeds the tile capacity falls back to reading the pairs
-//    straight from global memory, so any geometry stays correct;
+//    a projection whose box exceeds the tile capacity reads entries from global memory
+//    instead, so any geometry stays correct;
 //  - workgroups are numbered so that each XCD (dispatch is round-robin over 8 XCDs)
 //    receives a contiguous range of tiles and shares detector rows in its own L2.
 #include "ctg_internal.h"
@@ -29,12 +30,13 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 namespace ctg {
 namespace {
 
-constexpr int kTX = 16, kTY = 16, kThreads = 256;
-constexpr int kMaxE = 10;  // staged entries per thread: tile capacity kMaxE * 256
+constexpr int kTX = 16, kTY = 16, kThreads = 256, kWaves = kThreads / 64;
+constexpr int kMaxTileBytes = 36 * 1024;  // per LDS buffer (two buffers per workgroup)
 
 struct BpArgs {
     int W, H, nx, ny, nzs, z0;
@@ -43,41 +45,45 @@ struct BpArgs {
     int nproj, ntx, nty, nzb, cap, accumulate, vec4;
 };
 
+// Entry formats: FP32 (16 B), FP16 and BF16 (8 B: {A,dU} {dV,dUV} as 16-bit pairs).
 struct EF32 {
-    using T = float2;
-    __device__ static T zero() { return make_float2(0.0f, 0.0f); }
-    __device__ static void taps(T e0, T e1, float& A, float& B, float& C, float& D) {
-        A = e0.x; B = e0.y; C = e1.x; D = e1.y;
+    using T = float4;
+    static constexpr int kBytes = 16, kDma = 16;
+    __device__ static float eval(const T& e, float fu, float fv) {
+        return fmaf(fv, fmaf(fu, e.w, e.z), fmaf(fu, e.y, e.x));
     }
 };
 struct EF16 {
-    using T = uint32_t;
-    __device__ static T zero() { return 0u; }
+    using T = uint2;
+    static constexpr int kBytes = 8, kDma = 4;
     __device__ static float lo(uint32_t e) { return __half2float(__ushort_as_half((unsigned short)(e & 0xffffu))); }
     __device__ static float hi(uint32_t e) { return __half2float(__ushort_as_half((unsigned short)(e >> 16))); }
-    __device__ static void taps(T e0, T e1, float& A, float& B, float& C, float& D) {
-        A = lo(e0); B = hi(e0); C = lo(e1); D = hi(e1);
+    __device__ static float eval(const T& e, float fu, float fv) {
+        return fmaf(fv, fmaf(fu, hi(e.y), lo(e.y)), fmaf(fu, hi(e.x), lo(e.x)));
     }
 };
 struct EBF16 {
-    using T = uint32_t;
-    __device__ static T zero() { return 0u; }
-    __device__ static void taps(T e0, T e1, float& A, float& B, float& C, float& D) {
-        A = __uint_as_float(e0 << 16); B = __uint_as_float(e0 & 0xffff0000u);
-        C = __uint_as_float(e1 << 16); D = __uint_as_float(e1 & 0xffff0000u);
+    using T = uint2;
+    static constexpr int kBytes = 8, kDma = 4;
+    __device__ static float lo(uint32_t e) { return __uint_as_float(e << 16); }
+    __device__ static float hi(uint32_t e) { return __uint_as_float(e & 0xffff0000u); }
+    __device__ static float eval(const T& e, float fu, float fv) {
+        return fmaf(fv, fmaf(fu, hi(e.y), lo(e.y)), fmaf(fu, hi(e.x), lo(e.x)));
     }
 };
 
-__device__ inline float interp(float fu, float fv, float A, float B, float C, float D) {
-    float top = fmaf(fu, B, fmaf(-fu, A, A));
-    float bot = fmaf(fu, D, fmaf(-fu, C, C));
-    return fmaf(fv, bot - top, top);
+__device__ inline int cvt_flr(float x) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
 }
 
-// box: x = first pair column, y = first row (may be -1: zero guard row), z = ncols | flag<<28,
-// w = nrows.  flag 0: the block cannot hit the detector; 1: staged in LDS; 2: global reads.
-__device__ inline int box_flag(int4 b) { return (unsigned)b.z >> 28; }
-__device__ inline int box_nc(int4 b) { return b.z & 0x0fffffff; }
+// box: x = first entry column, y = first row, w = rows, z = ncols | flags.
+// bits 28-29: 0 the block cannot hit the detector, 1 staged in LDS, 2 global reads;
+// bit 27: the fast (unmasked) path may be used.
+__device__ inline int box_kind(int4 b) { return ((unsigned)b.z >> 28) & 3; }
+__device__ inline bool box_fast(int4 b) { return (b.z >> 27) & 1; }
+__device__ inline int box_nc(int4 b) { return b.z & 0x07ffffff; }
 
 template <int ZPL, class E>
 __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __restrict__ pp,
@@ -86,7 +92,6 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
                                                       float* __restrict__ vol) {
     using T = typename E::T;
     extern __shared__ __align__(16) unsigned char smem[];
-    T* tile = reinterpret_cast<T*>(smem);
     __shared__ int4 boxes[2][64];
 
     // XCD-aware numbering: dispatch sends workgroup L to XCD L%8; give each XCD a
@@ -99,6 +104,7 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
     const int txy = (int)(item - (unsigned)zbi * ntxy);
     const int ty = txy / a.ntx, tx = txy - ty * a.ntx;
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
     const int xi = tx * kTX + (tid & (kTX - 1));
     const int yi = ty * kTY + (tid / kTX);
     const int zb = zbi * ZPL;
@@ -150,48 +156,42 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
                 b.z = 0;
                 return b;
             }
-            const int us = max((int)floorf(umin) - 1, 0);
+            const int us0 = (int)floorf(umin) - 1, vs0 = (int)floorf(vmin) - 1;
+            const int us = max(us0, 0);
             const int ue = min((int)floorf(umax) + 1, a.W - 1);
-            const int vs = max((int)floorf(vmin) - 1, -1);
-            const int ve = min((int)floorf(vmax) + 2, a.H - 1);
+            const int vs = max(vs0, 0);
+            const int ve = min((int)floorf(vmax) + 1, a.H - 1);
             const int nc = ue - us + 1, nr = ve - vs + 1;
-            if (nc <= 0 || nr <= 1) { b.z = 0; return b; }
+            if (nc <= 0 || nr <= 0) { b.z = 0; return b; }
             b.x = us; b.y = vs; b.w = nr;
-            b.z = nc | ((nc * nr <= a.cap ? 1 : 2) << 28);
+            // the fused vIndex of the fast path may round a hair below 0 at the top edge
+            const int fast = vs0 >= 0 ? 1 : 0;
+            b.z = nc | (fast << 27) | ((nc * nr <= a.cap ? 1 : 2) << 28);
             return b;
         };
 
-        T regs[kMaxE];
-        auto issue = [&](int p, int4 bx) {
-            const int us = bx.x, vs = bx.y, nr = bx.w;
-            const int ne = box_nc(bx) * nr;
-            const T* src = proj + (size_t)p * a.W * a.H;
-            int c = tid / nr, rr = tid - (tid / nr) * nr;
-            const int qs = kThreads / nr, rs = kThreads - (kThreads / nr) * nr;
-#pragma unroll
-            for (int i = 0; i < kMaxE; ++i) {
-                const int e = tid + i * kThreads;
-                const int gv = vs + rr;
-                T val = E::zero();
-                if (e < ne && gv >= 0) val = src[(size_t)(us + c) * a.H + gv];
-                regs[i] = val;
-                c += qs;
-                rr += rs;
-                if (rr >= nr) { rr -= nr; ++c; }
-            }
-        };
-        auto commit = [&](int4 bx) {
-            const int ne = box_nc(bx) * bx.w;
-#pragma unroll
-            for (int i = 0; i < kMaxE; ++i) {
-                const int e = tid + i * kThreads;
-                if (e < ne) tile[e] = regs[i];
+        // Stage the box of projection p into LDS buffer `buf`: wave w copies detector
+        // columns w, w+4, ...; a column's entries are contiguous in global memory and in LDS.
+        auto stage = [&](int p, int4 bx, int buf) {
+            const int us = bx.x, vs = bx.y, nr = bx.w, nc = box_nc(bx);
+            const int units = nr * (E::kBytes / E::kDma);
+            const char* src = reinterpret_cast<const char*>(proj) +
+                              (((size_t)p * a.W + us) * a.H + vs) * E::kBytes;
+            unsigned char* dst = smem + (size_t)buf * a.cap * E::kBytes;
+            for (int c = wave; c < nc; c += kWaves) {
+                const char* gcol = src + (size_t)c * a.H * E::kBytes;
+                unsigned char* lcol = dst + (size_t)c * nr * E::kBytes;
+                for (int j = 0; j < units; j += 64) {
+                    if (j + lane < units)
+                        __builtin_amdgcn_global_load_lds(gcol + (size_t)(j + lane) * E::kDma,
+                                                         (void*)(lcol + (size_t)j * E::kDma), E::kDma, 0, 0);
+                }
             }
         };
 
-        auto compute = [&](int p, int4 bx) {
-            const int flag = box_flag(bx);
-            if (flag == 0) return;
+        auto compute = [&](int p, int4 bx, int buf) {
+            const int kind = box_kind(bx);
+            if (kind == 0) return;
             const float4 g = pp[p];
             const float sc = scales ? scales[p] : 1.0f;
             float t = (-1.0f) * x * g.x + y * g.y;
@@ -204,31 +204,28 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
             const int u0 = (int)uIdx;
             const float fu = uIdx - (float)u0;
             const float w = U * U * sc;
-            if (flag == 1) {
+            if (kind == 1) {
+                const T* tile = reinterpret_cast<const T*>(smem + (size_t)buf * a.cap * E::kBytes);
                 const int us = bx.x, vs = bx.y, nr = bx.w;
                 const bool lane_full =
                     !cv || (zcount == ZPL && (zw0 + g.z) * U >= a.vlo && (zw1 + g.z) * U <= a.vhi);
-                if (__all(lane_full)) {
-                    // every active column keeps all ZPL voxels inside the detector window
+                if (box_fast(bx) && __all(lane_full)) {
+                    // every active column keeps all ZPL voxels inside the detector window;
+                    // inactive columns read entry 0 with zero weight
                     const float Ue = cv ? U : 0.0f;
                     const float we = cv ? w : 0.0f;
                     const float fue = cv ? fu : 0.0f;
-                    // inactive columns read entry 0 with zero weight
                     const int base = cv ? (u0 - us) * nr - vs : -(int)floorf(a.vpre);
+                    const T* tb = tile + base;
                     const float vI0 = fmaf(-(zw0 + g.z), Ue, a.vpre);
 #pragma unroll
                     for (int k = 0; k < ZPL; ++k) {
                         const float vI = fmaf(-(float)k, Ue, vI0);
-                        const float fl = floorf(vI);
-                        const float fv = vI - fl;
-                        const int e = base + (int)fl;
-                        const T e0 = tile[e], e1 = tile[e + 1];
-                        float A, B, C, D;
-                        E::taps(e0, e1, A, B, C, D);
-                        acc[k] = fmaf(we, interp(fue, fv, A, B, C, D), acc[k]);
+                        const T e = tb[cvt_flr(vI)];
+                        acc[k] = fmaf(we, E::eval(e, fue, __builtin_amdgcn_fractf(vI)), acc[k]);
                     }
                 } else {
-                    const int base = (u0 - us) * nr - vs;
+                    const T* tb = tile + ((u0 - us) * nr - vs);
 #pragma unroll
                     for (int k = 0; k < ZPL; ++k) {
                         const float zk = zw0 + (float)k;
@@ -236,17 +233,13 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
                         if (cv && k < zcount && v >= a.vlo && v <= a.vhi) {
                             const float vI = (-1.0f) * v + a.vpre;
                             const int v0 = (int)vI;
-                            const float fv = vI - (float)v0;
-                            const int e = base + v0;
-                            const T e0 = tile[e], e1 = tile[e + 1];
-                            float A, B, C, D;
-                            E::taps(e0, e1, A, B, C, D);
-                            acc[k] = fmaf(w, interp(fu, fv, A, B, C, D), acc[k]);
+                            const T e = tb[v0];
+                            acc[k] = fmaf(w, E::eval(e, fu, vI - (float)v0), acc[k]);
                         }
                     }
                 }
             } else {
-                const T* src = proj + (size_t)p * a.W * a.H + (size_t)u0 * a.H;
+                const T* src = proj + ((size_t)p * a.W + u0) * a.H;
 #pragma unroll
                 for (int k = 0; k < ZPL; ++k) {
                     const float zk = zw0 + (float)k;
@@ -254,36 +247,38 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
                     if (cv && k < zcount && v >= a.vlo && v <= a.vhi) {
                         const float vI = (-1.0f) * v + a.vpre;
                         const int v0 = (int)vI;
-                        const float fv = vI - (float)v0;
-                        const T e0 = src[v0], e1 = src[v0 + 1];
-                        float A, B, C, D;
-                        E::taps(e0, e1, A, B, C, D);
-                        acc[k] = fmaf(w, interp(fu, fv, A, B, C, D), acc[k]);
+                        const T e = src[v0];
+                        acc[k] = fmaf(w, E::eval(e, fu, vI - (float)v0), acc[k]);
                     }
                 }
             }
         };
 
+        // Buffer p&1 holds projection p while it is computed.  The box of p+1 is always
+        // produced at least one barrier before it is staged (batches are computed one
+        // batch ahead by wave 0).
         if (tid < 64) boxes[0][tid] = tid < a.nproj ? box_for(tid) : make_int4(0, 0, 0, 0);
         __syncthreads();
-        {
+        if (a.nproj > 0) {
             const int4 b0 = boxes[0][0];
-            if (a.nproj > 0 && box_flag(b0) == 1) issue(0, b0);
+            if (box_kind(b0) == 1) stage(0, b0, 0);
         }
         for (int p = 0; p < a.nproj; ++p) {
             const int4 bx = boxes[(p >> 6) & 1][p & 63];
-            __syncthreads();  // tile and the other box buffer are free
+            // this wave's DMA for projection p has landed; the barrier publishes every
+            // wave's and guarantees that nobody still reads buffer (p+1)&1 or the box
+            // batch about to be overwritten
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+            __syncthreads();
             if ((p & 63) == 0 && tid < 64) {
                 const int pn = p + 64 + tid;
                 boxes[((p >> 6) + 1) & 1][tid] = pn < a.nproj ? box_for(pn) : make_int4(0, 0, 0, 0);
             }
-            if (box_flag(bx) == 1) commit(bx);
-            __syncthreads();  // tile of p and the next box batch are visible
             if (p + 1 < a.nproj) {
                 const int4 bn = boxes[((p + 1) >> 6) & 1][(p + 1) & 63];
-                if (box_flag(bn) == 1) issue(p + 1, bn);
+                if (box_kind(bn) == 1) stage(p + 1, bn, (p + 1) & 1);
             }
-            compute(p, bx);
+            compute(p, bx, p & 1);
         }
     }
 
@@ -385,70 +380,105 @@ BpArgs make_args(const ctg_geometry& g, int nproj, int z0, int nz, int accumulat
     return a;
 }
 
-constexpr int kZPL = 32;
-
-// Upper bound on the detector footprint (pair entries) of one 16x16xZPL block that
-// intersects the cylinder, for any projection angle.
-int footprint_bound(const ctg_geometry& g, int zpl) {
-    const double diag = 15.0 * std::sqrt(2.0) + 0.01;
-    const double Rt = (double)g.radius + diag + 1.0;
-    const double fcd = g.fcd;
-    if (fcd <= Rt + 1.0) return 0;
-    const double Umax = fcd / (fcd - Rt);
-    const double dU = fcd * diag / ((fcd - Rt) * (fcd - Rt));
-    const double tmax = Rt + std::fabs((double)g.u_offset);
-    const double urange = diag * Umax + tmax * dU;
-    const double zmax = std::max((double)g.z_pre, (double)g.nz - g.z_pre) + 1.0;
-    const double vrange = (zpl - 1) * Umax + zmax * dU;
-    const int ncols = (int)std::ceil(urange) + 4;
-    const int nrows = (int)std::ceil(vrange) + 5;
-    return ncols * nrows;
+// Largest footprint box (entries) over a sample of tiles and projections, with the
+// same construction as box_for on the device.  Boxes that exceed the resulting
+// capacity at run time take the global-memory path, so sampling only affects speed.
+int sample_max_box(const ctg_geometry& g, const float4* hp, int nproj, int z0, int nz, int zpl) {
+    const int ntx = (g.nx + kTX - 1) / kTX, nty = (g.ny + kTY - 1) / kTY, nzb = (nz + zpl - 1) / zpl;
+    const int pstep = std::max(1, nproj / 48);
+    int best = 0;
+    const int zblocks[3] = {0, nzb / 2, nzb - 1};
+    for (int zi = 0; zi < 3; ++zi) {
+        const int zb = zblocks[zi] * zpl;
+        const int zcount = std::min(zpl, nz - zb);
+        const float zw0 = (float)(z0 + zb) - g.z_pre, zw1 = zw0 + (float)(zcount - 1);
+        for (int ty = 0; ty < nty; ++ty) {
+            for (int tx = 0; tx < ntx; ++tx) {
+                const float cxa = (float)(tx * kTX) - g.x_pre;
+                const float cxb = (float)std::min(tx * kTX + kTX - 1, g.nx - 1) - g.x_pre;
+                const float cya = (float)(ty * kTY) - g.y_pre;
+                const float cyb = (float)std::min(ty * kTY + kTY - 1, g.ny - 1) - g.y_pre;
+                const float ndx = cxa > 0.0f ? cxa : (cxb < 0.0f ? cxb : 0.0f);
+                const float ndy = cya > 0.0f ? cya : (cyb < 0.0f ? cyb : 0.0f);
+                if (!(ndx * ndx + ndy * ndy < g.radius * g.radius)) continue;
+                for (int p = 0; p < nproj; p += pstep) {
+                    const float4 q = hp[p];
+                    float umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
+                    bool bad = false;
+                    for (int c = 0; c < 4; ++c) {
+                        const float cx = (c & 1) ? cxb : cxa, cy = (c & 2) ? cyb : cya;
+                        float t = (-1.0f) * cx * q.x + cy * q.y;
+                        t += g.u_offset;
+                        const float s = cx * q.y + cy * q.x;
+                        const float den = g.fcd - s;
+                        if (!(den > 0.0f)) bad = true;
+                        const float U = g.fcd / den;
+                        const float ui = t * U + g.u_pre;
+                        const float va = std::fma(-(zw0 + q.z), U, g.v_pre);
+                        const float vb = std::fma(-(zw1 + q.z), U, g.v_pre);
+                        umin = std::min(umin, ui); umax = std::max(umax, ui);
+                        vmin = std::min(vmin, std::min(va, vb)); vmax = std::max(vmax, std::max(va, vb));
+                    }
+                    if (bad) continue;
+                    const int us = std::max((int)std::floor(umin) - 1, 0);
+                    const int ue = std::min((int)std::floor(umax) + 1, g.det_width - 1);
+                    const int vs = std::max((int)std::floor(vmin) - 1, 0);
+                    const int ve = std::min((int)std::floor(vmax) + 1, g.det_height - 1);
+                    if (ue >= us && ve >= vs) best = std::max(best, (ue - us + 1) * (ve - vs + 1));
+                }
+            }
+        }
+    }
+    return best;
 }
 
-}  // namespace
-
-BpPlan plan_backproject(const ctg_geometry& g, ctg_store store, int nz) {
-    BpPlan pl{};
-    pl.zpl = kZPL;
-    const int ntx = (g.nx + kTX - 1) / kTX, nty = (g.ny + kTY - 1) / kTY, nzb = (nz + kZPL - 1) / kZPL;
-    pl.grid = ntx * nty * nzb;
-    int cap = std::min(footprint_bound(g, kZPL), kMaxE * kThreads);
-    if (cap < 2) cap = 2;
-    pl.cap_entries = cap;
-    pl.lds_bytes = cap * (store == CTG_STORE_F32 ? 8 : 4);
-    return pl;
-}
-
-ctg_status launch_backproject(const BpLaunch& L, hipStream_t st) {
+template <int ZPL>
+ctg_status launch_zpl(const BpLaunch& L, BpArgs a, hipStream_t st) {
     const ctg_geometry& g = *L.g;
-    if (L.nz <= 0 || g.nx <= 0 || g.ny <= 0) return CTG_OK;
-    BpArgs a = make_args(g, L.nproj, L.z0, L.nz, L.accumulate);
-    const BpPlan pl = plan_backproject(g, L.store, L.nz);
-    a.ntx = (g.nx + kTX - 1) / kTX;
-    a.nty = (g.ny + kTY - 1) / kTY;
-    a.nzb = (L.nz + kZPL - 1) / kZPL;
-    a.cap = pl.cap_entries;
-    a.vec4 = (L.nz % 4 == 0 && ((uintptr_t)L.d_volume % 16) == 0) ? 1 : 0;
+    const int ebytes = L.store == CTG_STORE_F32 ? 16 : 8;
+    a.nzb = (L.nz + ZPL - 1) / ZPL;
+    int cap = L.h_params ? sample_max_box(g, L.h_params, L.nproj, L.z0, L.nz, ZPL) : 0;
+    cap = cap + cap / 8 + 16;
+    cap = std::min(cap, kMaxTileBytes / ebytes);
+    a.cap = cap;
+    const size_t lds = (size_t)2 * cap * ebytes;
     const long long grid = (long long)a.ntx * a.nty * a.nzb;
     if (grid > 0x7fffffffll) return fail(CTG_ERROR_UNSUPPORTED, "backprojection grid too large");
     switch (L.store) {
         case CTG_STORE_F32:
-            hipLaunchKernelGGL((bp_kernel<kZPL, EF32>), dim3((unsigned)grid), dim3(kThreads), pl.lds_bytes, st, a,
-                               L.d_params, L.d_scales, (const float2*)L.d_filtered, L.d_volume);
+            hipLaunchKernelGGL((bp_kernel<ZPL, EF32>), dim3((unsigned)grid), dim3(kThreads), lds, st, a,
+                               L.d_params, L.d_scales, (const float4*)L.d_filtered, L.d_volume);
             break;
         case CTG_STORE_F16:
-            hipLaunchKernelGGL((bp_kernel<kZPL, EF16>), dim3((unsigned)grid), dim3(kThreads), pl.lds_bytes, st, a,
-                               L.d_params, L.d_scales, (const uint32_t*)L.d_filtered, L.d_volume);
+            hipLaunchKernelGGL((bp_kernel<ZPL, EF16>), dim3((unsigned)grid), dim3(kThreads), lds, st, a,
+                               L.d_params, L.d_scales, (const uint2*)L.d_filtered, L.d_volume);
             break;
         case CTG_STORE_BF16:
-            hipLaunchKernelGGL((bp_kernel<kZPL, EBF16>), dim3((unsigned)grid), dim3(kThreads), pl.lds_bytes, st, a,
-                               L.d_params, L.d_scales, (const uint32_t*)L.d_filtered, L.d_volume);
+            hipLaunchKernelGGL((bp_kernel<ZPL, EBF16>), dim3((unsigned)grid), dim3(kThreads), lds, st, a,
+                               L.d_params, L.d_scales, (const uint2*)L.d_filtered, L.d_volume);
             break;
         default:
             return fail(CTG_ERROR_INVALID_ARGUMENT, "unknown store type %d", (int)L.store);
     }
     CTG_HIP(hipGetLastError());
     return CTG_OK;
+}
+
+}  // namespace
+
+ctg_status launch_backproject(const BpLaunch& L, hipStream_t st) {
+    const ctg_geometry& g = *L.g;
+    if (L.nz <= 0 || g.nx <= 0 || g.ny <= 0) return CTG_OK;
+    if (((uintptr_t)L.d_filtered % 16) != 0)
+        return fail(CTG_ERROR_INVALID_ARGUMENT, "filtered projections must be 16-byte aligned");
+    BpArgs a = make_args(g, L.nproj, L.z0, L.nz, L.accumulate);
+    a.ntx = (g.nx + kTX - 1) / kTX;
+    a.nty = (g.ny + kTY - 1) / kTY;
+    a.vec4 = (L.nz % 4 == 0 && ((uintptr_t)L.d_volume % 16) == 0) ? 1 : 0;
+    // z per lane: 32 by default; CTG_BP_ZPL=16 selects the shorter run
+    const char* env = std::getenv("CTG_BP_ZPL");
+    const int zpl = (env && std::atoi(env) == 16) ? 16 : 32;
+    return zpl == 16 ? launch_zpl<16>(L, a, st) : launch_zpl<32>(L, a, st);
 }
 
 ctg_status launch_backproject_f64(const ctg_geometry& g, const double4* d_params, const double* d_rows,

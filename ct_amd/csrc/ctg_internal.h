@@ -100,6 +100,7 @@ ctg_status launch_layout(const ctg_geometry& g, const float* d_rows, int nproj, 
 struct BpLaunch {
     const ctg_geometry* g;
     const float4* d_params;  // nproj entries: sin, cos, height, unused
+    const float4* h_params;  // host copy of the same entries (tile sizing)
     const float* d_scales;   // nproj per-projection scales, or nullptr for 1
     const void* d_filtered;  // BP layout, nproj projections
     ctg_store store;
@@ -109,11 +110,6 @@ struct BpLaunch {
     int accumulate;
 };
 ctg_status launch_backproject(const BpLaunch& L, hipStream_t st);
-// Launch shape of the backprojection for a geometry: z per lane, LDS tile capacity.
-struct BpPlan {
-    int zpl, cap_entries, lds_bytes, grid;
-};
-BpPlan plan_backproject(const ctg_geometry& g, ctg_store store, int nz);
 ctg_status launch_backproject_f64(const ctg_geometry& g, const double4* d_params, const double* d_rows,
                                   int nproj, double* d_volume, int z0, int nz, hipStream_t st);
 

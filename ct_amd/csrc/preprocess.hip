@@ -229,9 +229,12 @@ __global__ __launch_bounds__(kRowThreads) void rows_kernel(PreArgs a, const Tin*
     }
 }
 
-// Row-major FP32 images -> backprojection layout: entry (u, v) of image p holds the
-// horizontal tap pair (q[v][u], q[v][u+1]) (q[v][W] := 0), stored column-major
-// (v fastest) so that a detector column is contiguous.
+// Row-major FP32 images -> backprojection layout.  Entry (u, v) of image p holds the
+// bilinear coefficients of the cell whose top-left texel is (u, v):
+//   A = q[v][u], dU = q[v][u+1] - A, dV = q[v+1][u] - A, dUV = q[v+1][u+1] - q[v+1][u] - dU
+// (texels past the last column or row are 0), so that the interpolated value at
+// (u + fu, v + fv) is A + fu*dU + fv*(dV + fu*dUV).  Entries are stored column-major
+// (v fastest): a detector column is contiguous, and one voxel-update reads one entry.
 __device__ inline uint32_t pack_f16(float a, float b) {
     __half ha = __float2half_rn(a), hb = __float2half_rn(b);
     return (uint32_t)__half_as_ushort(ha) | ((uint32_t)__half_as_ushort(hb) << 16);
@@ -247,11 +250,11 @@ template <int STORE>
 __global__ __launch_bounds__(256) void layout_kernel(const float* __restrict__ rows, int W, int H,
                                                      const unsigned* __restrict__ maxabs,
                                                      void* __restrict__ out, float* __restrict__ scales) {
-    __shared__ float tile[32][33];
+    __shared__ float tile[33][33];
     const int u0 = blockIdx.x * 32, v0 = blockIdx.y * 32;
     const size_t p = blockIdx.z;
     const float* img = rows + p * (size_t)W * H;
-    for (int i = threadIdx.x; i < 32 * 33; i += 256) {
+    for (int i = threadIdx.x; i < 33 * 33; i += 256) {
         int vl = i / 33, ul = i - vl * 33;
         int v = v0 + vl, u = u0 + ul;
         tile[vl][ul] = (v < H && u < W) ? img[(size_t)v * W + u] : 0.0f;
@@ -272,15 +275,16 @@ __global__ __launch_bounds__(256) void layout_kernel(const float* __restrict__ r
     for (int ul = threadIdx.x >> 5; ul < 32; ul += 8) {
         int u = u0 + ul, v = v0 + vl;
         if (u < W && v < H) {
-            float qa = tile[vl][ul] * inv;
-            float qb = (u + 1 < W) ? tile[vl][ul + 1] * inv : 0.0f;
+            const float A = tile[vl][ul] * inv, B = tile[vl][ul + 1] * inv;
+            const float C = tile[vl + 1][ul] * inv, D = tile[vl + 1][ul + 1] * inv;
+            const float dU = B - A, dV = C - A, dUV = (D - C) - dU;
             size_t idx = (p * (size_t)W + u) * H + v;
             if (STORE == CTG_STORE_F32) {
-                reinterpret_cast<float2*>(out)[idx] = make_float2(qa, qb);
+                reinterpret_cast<float4*>(out)[idx] = make_float4(A, dU, dV, dUV);
             } else if (STORE == CTG_STORE_F16) {
-                reinterpret_cast<uint32_t*>(out)[idx] = pack_f16(qa, qb);
+                reinterpret_cast<uint2*>(out)[idx] = make_uint2(pack_f16(A, dU), pack_f16(dV, dUV));
             } else {
-                reinterpret_cast<uint32_t*>(out)[idx] = pack_bf16(qa, qb);
+                reinterpret_cast<uint2*>(out)[idx] = make_uint2(pack_bf16(A, dU), pack_bf16(dV, dUV));
             }
         }
     }

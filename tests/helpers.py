@@ -29,6 +29,6 @@ def small_scan(W=64, H=48, P=24, fcd=None, seed=11, n_ell=6, photons=0.0, base=1
                 raw=I, lineint=p, ell=ell, base=base)
 
 
-def pairs_to_rows(pairs: np.ndarray, W: int, H: int) -> np.ndarray:
-    """Backprojection layout (P, W, H, 2) -> row-major images (P, H, W) from the first tap."""
-    return np.ascontiguousarray(np.transpose(pairs[..., 0], (0, 2, 1)))
+def entries_to_rows(entries: np.ndarray) -> np.ndarray:
+    """FP32 backprojection layout (P, W, H, 4) -> row-major images (P, H, W) (the A term)."""
+    return np.ascontiguousarray(np.transpose(entries[..., 0], (0, 2, 1)))

@@ -75,8 +75,8 @@ def test_filtered_bytes(ctg_lib):
     from ct_amd import _ctg
     g = ct_amd.geometry_grid(1024, 1024, 2500.0, 0.0, 8, 8, 8)
     f = ctg_lib.ctg_filtered_bytes
-    assert f(ctypes.byref(g), _ctg.STORE["f32"], 720) == 1024 * 1024 * 8 * 720
-    assert f(ctypes.byref(g), _ctg.STORE["f16"], 3) == 1024 * 1024 * 4 * 3
+    assert f(ctypes.byref(g), _ctg.STORE["f32"], 720) == 1024 * 1024 * 16 * 720
+    assert f(ctypes.byref(g), _ctg.STORE["f16"], 3) == 1024 * 1024 * 8 * 3
     assert f(ctypes.byref(g), _ctg.STORE["bf16"], 0) == 0
 
 

@@ -10,7 +10,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from helpers import pairs_to_rows, rel_rmse, small_scan
+from helpers import entries_to_rows, rel_rmse, small_scan
 
 pytestmark = pytest.mark.gpu
 
@@ -56,19 +56,25 @@ def test_preprocess_matches_oracle(gpu, oracle):
     for filt in ("ramlak", "shepplogan", "hann"):
         for log_form in ("cpu", "gpu"):
             raw = torch.from_numpy(scan["raw"]).cuda()
-            filtered = torch.empty((P, W, H, 2), dtype=torch.float32, device="cuda")
+            filtered = torch.empty((P, W, H, 4), dtype=torch.float32, device="cuda")
             scales = torch.empty(P, dtype=torch.float32, device="cuda")
             ctx.preprocess(raw, "u16", P, scan["base"], log_form, filt, "f32", filtered, scales)
             torch.cuda.synchronize()
-            got = pairs_to_rows(filtered.cpu().numpy(), W, H)
+            got = entries_to_rows(filtered.cpu().numpy())
             ref = oracle.preprocess_f32(_oracle_geom(g), scan["raw"], "u16", scan["base"], filt, log_form)
             err = rel_rmse(got, ref)
             print(f"preprocess {filt}/{log_form}: rel RMSE {err:.3e}")
             assert err < 2e-6, (filt, log_form, err)
-            # second tap of each pair is the right-hand neighbour, 0 past the last column
-            pr = filtered.cpu().numpy()
-            assert np.array_equal(pr[:, :-1, :, 1], pr[:, 1:, :, 0])
-            assert np.all(pr[:, -1, :, 1] == 0)
+            # entry (u,v) = (A, dU, dV, dUV) of the cell with top-left texel (u,v);
+            # texels past the last column/row are 0
+            e = filtered.cpu().numpy().astype(np.float64)
+            q = np.zeros((P, W + 1, H + 1))
+            q[:, :W, :H] = e[..., 0]
+            A, B, C, D = q[:, :W, :H], q[:, 1:, :H], q[:, :W, 1:], q[:, 1:, 1:]
+            scale = np.max(np.abs(A))
+            assert np.max(np.abs(e[..., 1] - (B - A))) < 1e-6 * scale
+            assert np.max(np.abs(e[..., 2] - (C - A))) < 1e-6 * scale
+            assert np.max(np.abs(e[..., 3] - (D - C - B + A))) < 2e-6 * scale
             assert np.all(scales.cpu().numpy() == 1.0)
 
 
