@@ -33,7 +33,7 @@
 #include <cstdlib>
 
 namespace ctg {
-namespace {
+namespace bp_detail {
 
 constexpr int kTX = 16, kTY = 16, kThreads = 256, kWaves = kThreads / 64;
 constexpr int kMaxTileBytes = 36 * 1024;  // per LDS buffer (two buffers per workgroup)
@@ -72,6 +72,20 @@ struct EBF16 {
     }
 };
 
+// Global->LDS DMA of `bytes` (4 or 16) per lane into lds + lane*bytes.  The builtin is
+// only visible to the device pass (the host pass rejects the gfx950-only 16-byte form
+// and would silently drop the kernel instantiation).
+__device__ inline void lds_dma16(const void* gsrc, void* lds_wave_base) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_global_load_lds(gsrc, lds_wave_base, 16, 0, 0);
+#endif
+}
+__device__ inline void lds_dma4(const void* gsrc, void* lds_wave_base) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_global_load_lds(gsrc, lds_wave_base, 4, 0, 0);
+#endif
+}
+
 __device__ inline int cvt_flr(float x) {
     int r;
     asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
@@ -88,9 +102,10 @@ __device__ inline int box_nc(int4 b) { return b.z & 0x07ffffff; }
 template <int ZPL, class E>
 __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __restrict__ pp,
                                                       const float* __restrict__ scales,
-                                                      const typename E::T* __restrict__ proj,
+                                                      const void* __restrict__ proj_raw,
                                                       float* __restrict__ vol) {
     using T = typename E::T;
+    const T* __restrict__ proj = reinterpret_cast<const T*>(proj_raw);
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int4 boxes[2][64];
 
@@ -182,9 +197,12 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
                 const char* gcol = src + (size_t)c * a.H * E::kBytes;
                 unsigned char* lcol = dst + (size_t)c * nr * E::kBytes;
                 for (int j = 0; j < units; j += 64) {
-                    if (j + lane < units)
-                        __builtin_amdgcn_global_load_lds(gcol + (size_t)(j + lane) * E::kDma,
-                                                         (void*)(lcol + (size_t)j * E::kDma), E::kDma, 0, 0);
+                    if (j + lane < units) {
+                        const void* g = gcol + (size_t)(j + lane) * E::kDma;
+                        void* l = (void*)(lcol + (size_t)j * E::kDma);
+                        if constexpr (E::kDma == 16) lds_dma16(g, l);
+                        else lds_dma4(g, l);
+                    }
                 }
             }
         };
@@ -447,15 +465,15 @@ ctg_status launch_zpl(const BpLaunch& L, BpArgs a, hipStream_t st) {
     switch (L.store) {
         case CTG_STORE_F32:
             hipLaunchKernelGGL((bp_kernel<ZPL, EF32>), dim3((unsigned)grid), dim3(kThreads), lds, st, a,
-                               L.d_params, L.d_scales, (const float4*)L.d_filtered, L.d_volume);
+                               L.d_params, L.d_scales, L.d_filtered, L.d_volume);
             break;
         case CTG_STORE_F16:
             hipLaunchKernelGGL((bp_kernel<ZPL, EF16>), dim3((unsigned)grid), dim3(kThreads), lds, st, a,
-                               L.d_params, L.d_scales, (const uint2*)L.d_filtered, L.d_volume);
+                               L.d_params, L.d_scales, L.d_filtered, L.d_volume);
             break;
         case CTG_STORE_BF16:
             hipLaunchKernelGGL((bp_kernel<ZPL, EBF16>), dim3((unsigned)grid), dim3(kThreads), lds, st, a,
-                               L.d_params, L.d_scales, (const uint2*)L.d_filtered, L.d_volume);
+                               L.d_params, L.d_scales, L.d_filtered, L.d_volume);
             break;
         default:
             return fail(CTG_ERROR_INVALID_ARGUMENT, "unknown store type %d", (int)L.store);
@@ -464,7 +482,9 @@ ctg_status launch_zpl(const BpLaunch& L, BpArgs a, hipStream_t st) {
     return CTG_OK;
 }
 
-}  // namespace
+}  // namespace bp_detail
+
+using namespace bp_detail;
 
 ctg_status launch_backproject(const BpLaunch& L, hipStream_t st) {
     const ctg_geometry& g = *L.g;
