@@ -98,6 +98,8 @@ __device__ inline int cvt_flr(float x) {
 __device__ inline int box_kind(int4 b) { return ((unsigned)b.z >> 28) & 3; }
 __device__ inline bool box_fast(int4 b) { return (b.z >> 27) & 1; }
 __device__ inline int box_nc(int4 b) { return b.z & 0x07ffffff; }
+__device__ inline int box_nr(int4 b) { return b.w & 0xffff; }
+__device__ inline int box_pitch(int4 b) { return (unsigned)b.w >> 16; }
 
 template <int ZPL, class E>
 __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __restrict__ pp,
@@ -132,8 +134,10 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
     const float cxb = (float)min(tx * kTX + kTX - 1, a.nx - 1) - a.xpre;
     const float cya = (float)(ty * kTY) - a.ypre;
     const float cyb = (float)min(ty * kTY + kTY - 1, a.ny - 1) - a.ypre;
+    // the z-run always spans ZPL voxels (those past the slab are computed, never stored),
+    // so boxes and the fast-path test cover the full run
     const float zw0 = (float)(a.z0 + zb) - a.zpre;
-    const float zw1 = zw0 + (float)(zcount - 1);
+    const float zw1 = zw0 + (float)(ZPL - 1);
 
     float acc[ZPL];
 #pragma unroll
@@ -178,24 +182,27 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
             const int ve = min((int)floorf(vmax) + 1, a.H - 1);
             const int nc = ue - us + 1, nr = ve - vs + 1;
             if (nc <= 0 || nr <= 0) { b.z = 0; return b; }
-            b.x = us; b.y = vs; b.w = nr;
+            // LDS column pitch: rows rounded up to 1 mod 16, so that neighbouring detector
+            // columns start in different LDS bank groups
+            const int pitch = nr + ((17 - (nr & 15)) & 15);
+            b.x = us; b.y = vs; b.w = nr | (pitch << 16);
             // the fused vIndex of the fast path may round a hair below 0 at the top edge
             const int fast = vs0 >= 0 ? 1 : 0;
-            b.z = nc | (fast << 27) | ((nc * nr <= a.cap ? 1 : 2) << 28);
+            b.z = nc | (fast << 27) | ((nc * pitch <= a.cap ? 1 : 2) << 28);
             return b;
         };
 
         // Stage the box of projection p into LDS buffer `buf`: wave w copies detector
         // columns w, w+4, ...; a column's entries are contiguous in global memory and in LDS.
         auto stage = [&](int p, int4 bx, int buf) {
-            const int us = bx.x, vs = bx.y, nr = bx.w, nc = box_nc(bx);
+            const int us = bx.x, vs = bx.y, nr = box_nr(bx), nc = box_nc(bx), pitch = box_pitch(bx);
             const int units = nr * (E::kBytes / E::kDma);
             const char* src = reinterpret_cast<const char*>(proj) +
                               (((size_t)p * a.W + us) * a.H + vs) * E::kBytes;
             unsigned char* dst = smem + (size_t)buf * a.cap * E::kBytes;
             for (int c = wave; c < nc; c += kWaves) {
                 const char* gcol = src + (size_t)c * a.H * E::kBytes;
-                unsigned char* lcol = dst + (size_t)c * nr * E::kBytes;
+                unsigned char* lcol = dst + (size_t)c * pitch * E::kBytes;
                 for (int j = 0; j < units; j += 64) {
                     if (j + lane < units) {
                         const void* g = gcol + (size_t)(j + lane) * E::kDma;
@@ -215,7 +222,9 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
             float t = (-1.0f) * x * g.x + y * g.y;
             t += a.uoff;
             const float s = x * g.y + y * g.x;
-            const float U = a.fcd / (a.fcd - s);
+            // v_rcp_f32 (1 ulp) instead of the IEEE division sequence; the reference's GPU
+            // kernel also divides with fast math
+            const float U = a.fcd * __builtin_amdgcn_rcpf(a.fcd - s);
             const float u = t * U;
             const bool cv = incyl && u >= a.ulo && u <= a.uhi;
             const float uIdx = u + a.upre;
@@ -224,9 +233,8 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
             const float w = U * U * sc;
             if (kind == 1) {
                 const T* tile = reinterpret_cast<const T*>(smem + (size_t)buf * a.cap * E::kBytes);
-                const int us = bx.x, vs = bx.y, nr = bx.w;
-                const bool lane_full =
-                    !cv || (zcount == ZPL && (zw0 + g.z) * U >= a.vlo && (zw1 + g.z) * U <= a.vhi);
+                const int us = bx.x, vs = bx.y, nr = box_pitch(bx);
+                const bool lane_full = !cv || ((zw0 + g.z) * U >= a.vlo && (zw1 + g.z) * U <= a.vhi);
                 if (box_fast(bx) && __all(lane_full)) {
                     // every active column keeps all ZPL voxels inside the detector window;
                     // inactive columns read entry 0 with zero weight
@@ -248,7 +256,7 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
                     for (int k = 0; k < ZPL; ++k) {
                         const float zk = zw0 + (float)k;
                         const float v = (zk + g.z) * U;
-                        if (cv && k < zcount && v >= a.vlo && v <= a.vhi) {
+                        if (cv && v >= a.vlo && v <= a.vhi) {
                             const float vI = (-1.0f) * v + a.vpre;
                             const int v0 = (int)vI;
                             const T e = tb[v0];
@@ -262,7 +270,7 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
                 for (int k = 0; k < ZPL; ++k) {
                     const float zk = zw0 + (float)k;
                     const float v = (zk + g.z) * U;
-                    if (cv && k < zcount && v >= a.vlo && v <= a.vhi) {
+                    if (cv && v >= a.vlo && v <= a.vhi) {
                         const float vI = (-1.0f) * v + a.vpre;
                         const int v0 = (int)vI;
                         const T e = src[v0];
@@ -281,8 +289,12 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
             const int4 b0 = boxes[0][0];
             if (box_kind(b0) == 1) stage(0, b0, 0);
         }
+        auto uniform_box = [](int4 b) {
+            return make_int4(__builtin_amdgcn_readfirstlane(b.x), __builtin_amdgcn_readfirstlane(b.y),
+                             __builtin_amdgcn_readfirstlane(b.z), __builtin_amdgcn_readfirstlane(b.w));
+        };
         for (int p = 0; p < a.nproj; ++p) {
-            const int4 bx = boxes[(p >> 6) & 1][p & 63];
+            const int4 bx = uniform_box(boxes[(p >> 6) & 1][p & 63]);
             // this wave's DMA for projection p has landed; the barrier publishes every
             // wave's and guarantees that nobody still reads buffer (p+1)&1 or the box
             // batch about to be overwritten
@@ -293,7 +305,7 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
                 boxes[((p >> 6) + 1) & 1][tid] = pn < a.nproj ? box_for(pn) : make_int4(0, 0, 0, 0);
             }
             if (p + 1 < a.nproj) {
-                const int4 bn = boxes[((p + 1) >> 6) & 1][(p + 1) & 63];
+                const int4 bn = uniform_box(boxes[((p + 1) >> 6) & 1][(p + 1) & 63]);
                 if (box_kind(bn) == 1) stage(p + 1, bn, (p + 1) & 1);
             }
             compute(p, bx, p & 1);
@@ -408,8 +420,7 @@ int sample_max_box(const ctg_geometry& g, const float4* hp, int nproj, int z0, i
     const int zblocks[3] = {0, nzb / 2, nzb - 1};
     for (int zi = 0; zi < 3; ++zi) {
         const int zb = zblocks[zi] * zpl;
-        const int zcount = std::min(zpl, nz - zb);
-        const float zw0 = (float)(z0 + zb) - g.z_pre, zw1 = zw0 + (float)(zcount - 1);
+        const float zw0 = (float)(z0 + zb) - g.z_pre, zw1 = zw0 + (float)(zpl - 1);
         for (int ty = 0; ty < nty; ++ty) {
             for (int tx = 0; tx < ntx; ++tx) {
                 const float cxa = (float)(tx * kTX) - g.x_pre;
@@ -442,7 +453,11 @@ int sample_max_box(const ctg_geometry& g, const float4* hp, int nproj, int z0, i
                     const int ue = std::min((int)std::floor(umax) + 1, g.det_width - 1);
                     const int vs = std::max((int)std::floor(vmin) - 1, 0);
                     const int ve = std::min((int)std::floor(vmax) + 1, g.det_height - 1);
-                    if (ue >= us && ve >= vs) best = std::max(best, (ue - us + 1) * (ve - vs + 1));
+                    if (ue >= us && ve >= vs) {
+                        const int nr = ve - vs + 1;
+                        const int pitch = nr + ((17 - (nr & 15)) & 15);
+                        best = std::max(best, (ue - us + 1) * pitch);
+                    }
                 }
             }
         }
@@ -456,7 +471,7 @@ ctg_status launch_zpl(const BpLaunch& L, BpArgs a, hipStream_t st) {
     const int ebytes = L.store == CTG_STORE_F32 ? 16 : 8;
     a.nzb = (L.nz + ZPL - 1) / ZPL;
     int cap = L.h_params ? sample_max_box(g, L.h_params, L.nproj, L.z0, L.nz, ZPL) : 0;
-    cap = cap + cap / 8 + 16;
+    cap = cap + cap / 32 + 16;
     cap = std::min(cap, kMaxTileBytes / ebytes);
     a.cap = cap;
     const size_t lds = (size_t)2 * cap * ebytes;
