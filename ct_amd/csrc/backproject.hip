@@ -194,18 +194,23 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
 
         // Stage the box of projection p into LDS buffer `buf`: wave w copies detector
         // columns w, w+4, ...; a column's entries are contiguous in global memory and in LDS.
+        // Stage the box of projection p into LDS buffer `buf`: wave w copies detector
+        // columns w, w+4, ...; a column's entries are contiguous in global memory and in
+        // LDS.  All addresses are wave-uniform (SGPR arithmetic); each lane moves kDma bytes.
+        const int wave_u = __builtin_amdgcn_readfirstlane(wave);
         auto stage = [&](int p, int4 bx, int buf) {
             const int us = bx.x, vs = bx.y, nr = box_nr(bx), nc = box_nc(bx), pitch = box_pitch(bx);
             const int units = nr * (E::kBytes / E::kDma);
             const char* src = reinterpret_cast<const char*>(proj) +
                               (((size_t)p * a.W + us) * a.H + vs) * E::kBytes;
             unsigned char* dst = smem + (size_t)buf * a.cap * E::kBytes;
-            for (int c = wave; c < nc; c += kWaves) {
+            const unsigned lane_off = (unsigned)lane * E::kDma;
+            for (int c = wave_u; c < nc; c += kWaves) {
                 const char* gcol = src + (size_t)c * a.H * E::kBytes;
                 unsigned char* lcol = dst + (size_t)c * pitch * E::kBytes;
                 for (int j = 0; j < units; j += 64) {
-                    if (j + lane < units) {
-                        const void* g = gcol + (size_t)(j + lane) * E::kDma;
+                    if (lane < units - j) {
+                        const void* g = gcol + (size_t)j * E::kDma + lane_off;
                         void* l = (void*)(lcol + (size_t)j * E::kDma);
                         if constexpr (E::kDma == 16) lds_dma16(g, l);
                         else lds_dma4(g, l);
