@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 
 namespace ctg {
@@ -43,6 +44,8 @@ struct BpArgs {
     float xpre, ypre, zpre, upre, vpre;
     float r2, fcd, uoff, ulo, uhi, vlo, vhi;
     int nproj, ntx, nty, nzb, cap, accumulate, vec4;
+    int dma16;               // 8-byte entries staged with 16-byte DMA (H even: rows start even)
+    int pitch_mod, pitch_rem;  // LDS column pitch = rows rounded up to pitch_rem mod pitch_mod
 };
 
 // Entry formats: FP32 (16 B), FP16 and BF16 (8 B: {A,dU} {dV,dUV} as 16-bit pairs).
@@ -178,13 +181,13 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
             const int us0 = (int)floorf(umin) - 1, vs0 = (int)floorf(vmin) - 1;
             const int us = max(us0, 0);
             const int ue = min((int)floorf(umax) + 1, a.W - 1);
-            const int vs = max(vs0, 0);
+            const int vs = max(vs0, 0) & (a.dma16 ? ~1 : ~0);
             const int ve = min((int)floorf(vmax) + 1, a.H - 1);
             const int nc = ue - us + 1, nr = ve - vs + 1;
             if (nc <= 0 || nr <= 0) { b.z = 0; return b; }
-            // LDS column pitch: rows rounded up to 1 mod 16, so that neighbouring detector
-            // columns start in different LDS bank groups
-            const int pitch = nr + ((17 - (nr & 15)) & 15);
+            // LDS column pitch: rows rounded up to pitch_rem mod pitch_mod, so that
+            // neighbouring detector columns start in different LDS bank groups
+            const int pitch = nr + ((a.pitch_rem - nr) % a.pitch_mod + a.pitch_mod) % a.pitch_mod;
             b.x = us; b.y = vs; b.w = nr | (pitch << 16);
             // the fused vIndex of the fast path may round a hair below 0 at the top edge
             const int fast = vs0 >= 0 ? 1 : 0;
@@ -200,19 +203,21 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
         const int wave_u = __builtin_amdgcn_readfirstlane(wave);
         auto stage = [&](int p, int4 bx, int buf) {
             const int us = bx.x, vs = bx.y, nr = box_nr(bx), nc = box_nc(bx), pitch = box_pitch(bx);
-            const int units = nr * (E::kBytes / E::kDma);
+            const bool wide = E::kBytes == 16 || a.dma16;
+            const int ub = wide ? 16 : 4;  // bytes per lane per DMA
+            const int units = wide ? (nr * E::kBytes + 15) / 16 : nr * (E::kBytes / 4);
             const char* src = reinterpret_cast<const char*>(proj) +
                               (((size_t)p * a.W + us) * a.H + vs) * E::kBytes;
             unsigned char* dst = smem + (size_t)buf * a.cap * E::kBytes;
-            const unsigned lane_off = (unsigned)lane * E::kDma;
+            const unsigned lane_off = (unsigned)lane * ub;
             for (int c = wave_u; c < nc; c += kWaves) {
                 const char* gcol = src + (size_t)c * a.H * E::kBytes;
                 unsigned char* lcol = dst + (size_t)c * pitch * E::kBytes;
                 for (int j = 0; j < units; j += 64) {
                     if (lane < units - j) {
-                        const void* g = gcol + (size_t)j * E::kDma + lane_off;
-                        void* l = (void*)(lcol + (size_t)j * E::kDma);
-                        if constexpr (E::kDma == 16) lds_dma16(g, l);
+                        const void* g = gcol + (size_t)j * ub + lane_off;
+                        void* l = (void*)(lcol + (size_t)j * ub);
+                        if (wide) lds_dma16(g, l);
                         else lds_dma4(g, l);
                     }
                 }
@@ -418,7 +423,8 @@ BpArgs make_args(const ctg_geometry& g, int nproj, int z0, int nz, int accumulat
 // Largest footprint box (entries) over a sample of tiles and projections, with the
 // same construction as box_for on the device.  Boxes that exceed the resulting
 // capacity at run time take the global-memory path, so sampling only affects speed.
-int sample_max_box(const ctg_geometry& g, const float4* hp, int nproj, int z0, int nz, int zpl) {
+int sample_max_box(const ctg_geometry& g, const float4* hp, int nproj, int z0, int nz, int zpl, int dma16,
+                   int pmod, int prem) {
     const int ntx = (g.nx + kTX - 1) / kTX, nty = (g.ny + kTY - 1) / kTY, nzb = (nz + zpl - 1) / zpl;
     const int pstep = std::max(1, nproj / 48);
     int best = 0;
@@ -456,11 +462,11 @@ int sample_max_box(const ctg_geometry& g, const float4* hp, int nproj, int z0, i
                     if (bad) continue;
                     const int us = std::max((int)std::floor(umin) - 1, 0);
                     const int ue = std::min((int)std::floor(umax) + 1, g.det_width - 1);
-                    const int vs = std::max((int)std::floor(vmin) - 1, 0);
+                    const int vs = std::max((int)std::floor(vmin) - 1, 0) & (dma16 ? ~1 : ~0);
                     const int ve = std::min((int)std::floor(vmax) + 1, g.det_height - 1);
                     if (ue >= us && ve >= vs) {
                         const int nr = ve - vs + 1;
-                        const int pitch = nr + ((17 - (nr & 15)) & 15);
+                        const int pitch = nr + ((prem - nr) % pmod + pmod) % pmod;
                         best = std::max(best, (ue - us + 1) * pitch);
                     }
                 }
@@ -475,9 +481,22 @@ ctg_status launch_zpl(const BpLaunch& L, BpArgs a, hipStream_t st) {
     const ctg_geometry& g = *L.g;
     const int ebytes = L.store == CTG_STORE_F32 ? 16 : 8;
     a.nzb = (L.nz + ZPL - 1) / ZPL;
-    int cap = L.h_params ? sample_max_box(g, L.h_params, L.nproj, L.z0, L.nz, ZPL) : 0;
+    a.dma16 = (ebytes == 8 && g.det_height % 2 == 0) ? 1 : 0;
+    // column pitch: 1 mod 16 for 16-byte entries; 8-byte entries staged 16 bytes at a time
+    // need an even pitch (2 mod 16).  CTG_BP_PITCH="mod,rem" overrides (tuning).
+    a.pitch_mod = 16;
+    a.pitch_rem = (ebytes == 8 && a.dma16) ? 2 : 1;
+    if (const char* env = std::getenv("CTG_BP_PITCH")) {
+        int m = 0, r = 0;
+        if (std::sscanf(env, "%d,%d", &m, &r) == 2 && m >= 1 && r >= 0 && (!a.dma16 || (m % 2 == 0 && r % 2 == 0))) {
+            a.pitch_mod = m;
+            a.pitch_rem = r % m;
+        }
+    }
+    int cap = L.h_params ? sample_max_box(g, L.h_params, L.nproj, L.z0, L.nz, ZPL, a.dma16, a.pitch_mod, a.pitch_rem) : 0;
     cap = cap + cap / 32 + 16;
     cap = std::min(cap, kMaxTileBytes / ebytes);
+    cap &= ~7;  // keeps the second buffer and every column start 16-byte aligned
     a.cap = cap;
     const size_t lds = (size_t)2 * cap * ebytes;
     const long long grid = (long long)a.ntx * a.nty * a.nzb;
