@@ -27,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, 8 TB/s
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "backproject_traffic_1024cubed_720proj.json")
 BYTES_PER_UPDATE = 8.0     # one FP32 voxel load + one store per (voxel, projection)
 
 
@@ -208,6 +209,11 @@ def main():
 
     t32, bp32 = res["f32"]
     achieved = BYTES_PER_UPDATE * updates / bp32 / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE) and (args.det, args.nproj, args.grid) == (1024, 720, 1024):
+        # HBM bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated
+        # against a copy kernel of known size (see the profile summary beside the file)
+        traffic = json.load(open(TRAFFIC_FILE))["traffic_bytes_per_launch"]
     out = {
         "metric": "FDK voxel-update rate (backprojection GUPS), 1024^3 from 720x1024^2; FP32 vs FP16-stored projections",
         "value": updates / t32 / 1e9,
@@ -228,7 +234,7 @@ def main():
         "bp_ms_f32": bp32 * 1e3,
         "bp_gups_f32": updates / bp32 / 1e9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
     }
     if "f16" in res:

@@ -481,11 +481,15 @@ ctg_status launch_zpl(const BpLaunch& L, BpArgs a, hipStream_t st) {
     const ctg_geometry& g = *L.g;
     const int ebytes = L.store == CTG_STORE_F32 ? 16 : 8;
     a.nzb = (L.nz + ZPL - 1) / ZPL;
-    a.dma16 = (ebytes == 8 && g.det_height % 2 == 0) ? 1 : 0;
-    // column pitch: 1 mod 16 for 16-byte entries; 8-byte entries staged 16 bytes at a time
-    // need an even pitch (2 mod 16).  CTG_BP_PITCH="mod,rem" overrides (tuning).
+    // 8-byte entries are staged 4 bytes per lane by default: the 16-byte form needs an
+    // even LDS pitch, which measured slower (more bank conflicts on the 8-byte reads) than
+    // the odd pitch.  CTG_BP_DMA16=1 selects it (needs an even detector height).
+    a.dma16 = (ebytes == 8 && g.det_height % 2 == 0 && std::getenv("CTG_BP_DMA16") &&
+               std::atoi(std::getenv("CTG_BP_DMA16")) == 1) ? 1 : 0;
+    // column pitch: rows rounded up to 1 mod 16 (2 mod 16 for the 16-byte form of 8-byte
+    // entries).  CTG_BP_PITCH="mod,rem" overrides (tuning).
     a.pitch_mod = 16;
-    a.pitch_rem = (ebytes == 8 && a.dma16) ? 2 : 1;
+    a.pitch_rem = a.dma16 ? 2 : 1;
     if (const char* env = std::getenv("CTG_BP_PITCH")) {
         int m = 0, r = 0;
         if (std::sscanf(env, "%d,%d", &m, &r) == 2 && m >= 1 && r >= 0 && (!a.dma16 || (m % 2 == 0 && r % 2 == 0))) {
