@@ -35,6 +35,10 @@ def _oracle_volume(oracle, scan, pixel_size, filt, log_form, bounds=None, step=1
 def test_cli_zfastest_shepplogan_cpu_formulas(gpu, oracle, tmp_path):
     scan = small_scan(W=64, H=48, P=36, seed=31)
     csv = write_scan(str(tmp_path), scan["raw"], scan["angles"], pixel_size=0.5, fcd_px=scan["fcd"])
+    # like the reference writer (Ct/CtVolume.cpp:641-645), the output directory is not created
+    r = _run(["-i", csv, "-o", "out/vol.raw", "-j", "zfastest"], str(tmp_path))
+    assert r.returncode != 0 and "Could not write the info file." in r.stdout
+    os.makedirs(str(tmp_path / "out"))
     r = _run(["-i", csv, "-o", "out/vol.raw", "-f", "shepplogan", "-j", "zfastest", "-c"], str(tmp_path))
     assert r.returncode == 0, r.stdout
     ref, g = _oracle_volume(oracle, scan, 0.5, "shepplogan", "cpu")
