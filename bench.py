@@ -97,6 +97,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--only", choices=["f32", "f16", "both"], default="both")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end run from pinned host memory")
+    ap.add_argument("--no-roi", action="store_true", help="skip the region-of-interest run")
     ap.add_argument("--calib-copy-gib", type=float, default=0.0,
                     help="run a device-to-device copy of this size first (HBM counter calibration)")
     args = ap.parse_args()
@@ -162,6 +164,57 @@ def main():
             f"({updates / bp / 1e9:.1f} GUPS)")
         return t, bp, filtered, scales
 
+    def e2e_from_pinned_host(store, chunks=8):
+        """H2D of the raw uint16 projections from pinned host memory in chunks on an upload
+        stream, each chunk preprocessed on the compute stream as soon as it lands, then
+        the backprojection: the whole reconstruction from host images to the volume in HBM."""
+        filtered, scales = make_bufs(store)
+        raw16 = raw.view(torch.int16)  # same bytes; int16 has the full set of copy kernels
+        host = torch.empty(raw.shape, dtype=torch.int16, pin_memory=True)
+        host.copy_(raw16)
+        dev_raw = torch.empty_like(raw16)
+        per = ctx.filtered_bytes(store, 1)
+        up = torch.cuda.Stream(device=dev)
+        cmp = torch.cuda.current_stream()
+        bounds = [P * i // chunks for i in range(chunks + 1)]
+
+        def run():
+            evs = []
+            up.wait_stream(cmp)  # the previous run's preprocessing has finished reading dev_raw
+            with torch.cuda.stream(up):
+                for i in range(chunks):
+                    a, b = bounds[i], bounds[i + 1]
+                    dev_raw[a:b].copy_(host[a:b], non_blocking=True)
+                    e = torch.cuda.Event()
+                    e.record(up)
+                    evs.append(e)
+            for i in range(chunks):
+                a, b = bounds[i], bounds[i + 1]
+                cmp.wait_event(evs[i])
+                ctx.preprocess(dev_raw[a:b], "u16", b - a, base, "cpu", filt, store, filtered[a * per:b * per],
+                               scales[a:b], cmp.cuda_stream)
+            ctx.backproject(filtered, store, scales, 0, P, vol, 0, g.nz, False, cmp.cuda_stream)
+
+        run()
+        torch.cuda.synchronize()
+        reps = max(1, args.steps)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            run()
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / reps
+        h2d = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(up):
+            h2d[0].record(up)
+            dev_raw.copy_(host, non_blocking=True)
+            h2d[1].record(up)
+        torch.cuda.synchronize()
+        th2d = h2d[0].elapsed_time(h2d[1]) / 1e3
+        log(f"{store} end to end from pinned host: {t * 1e3:.1f} ms (H2D alone {th2d * 1e3:.1f} ms, "
+            f"{raw.numel() * 2 / th2d / 1e9:.1f} GB/s)")
+        del host, dev_raw, filtered, scales
+        return t, th2d
+
     res = {}
     if args.only == "f16":
         t16, bp16, _, _ = run_store("f16")
@@ -169,6 +222,7 @@ def main():
         return
     t32, bp32, filt32, sc32 = run_store("f32")
     res["f32"] = (t32, bp32)
+    e2e = None if args.no_e2e else e2e_from_pinned_host("f32")
     vol32 = vol.clone() if not args.no_precision else None
     rows_for_cpu = None
     if not args.no_cpu_baseline:
@@ -241,8 +295,44 @@ def main():
         t16, bp16 = res["f16"]
         out.update({"ms_per_recon_f16": t16 * 1e3, "bp_ms_f16": bp16 * 1e3, "bp_gups_f16": updates / bp16 / 1e9,
                     "f16_over_f32_bp": bp32 / bp16, "f16_over_f32_step": t32 / t16})
+    if e2e is not None:
+        out["ms_e2e_from_pinned_host_f32"] = e2e[0] * 1e3
+        out["ms_h2d_raw"] = e2e[1] * 1e3
+    if not args.no_roi:
+        out.update(roi_run(ctx, torch, raw, P, base, filt, stream, args.steps))
     out.update(prec)
     print(json.dumps(out), flush=True)
+
+
+def roi_run(ctx, torch, raw, P, base, filt, stream, steps):
+    """The same scan reconstructed on CtVolume's own grid (2R x 2R x H) restricted to
+    x, y in [0.25, 0.75] and z in [0.1, 0.9] with setVolumeBounds semantics."""
+    import ct_amd
+    g0 = ctx.geometry
+    g, roi = ct_amd.geometry_ctvolume(g0.det_width, g0.det_height, g0.fcd, g0.u_offset,
+                                      [0.25, 0.75, 0.25, 0.75, 0.1, 0.9])
+    ctx.set_geometry(g)
+    dev = raw.device
+    filtered = torch.empty(ctx.filtered_bytes("f32", P), dtype=torch.uint8, device=dev)
+    scales = torch.empty(P, dtype=torch.float32, device=dev)
+    vol = torch.empty((g.nx, g.ny, g.nz), dtype=torch.float32, device=dev)
+
+    def step():
+        ctx.preprocess(raw, "u16", P, base, "cpu", filt, "f32", filtered, scales, stream)
+        ctx.backproject(filtered, "f32", scales, 0, P, vol, 0, g.nz, False, stream)
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(max(1, steps)):
+        step()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / max(1, steps)
+    ups = float(g.nx) * g.ny * g.nz * P
+    log(f"ROI {g.nx}x{g.ny}x{g.nz} (x {roi[0]}..{roi[1]}, z {roi[4]}..{roi[5]}): {t * 1e3:.1f} ms, "
+        f"{ups / t / 1e9:.1f} GUPS")
+    ctx.set_geometry(g0)
+    return {"roi_grid": [g.nx, g.ny, g.nz], "roi_ms_f32": t * 1e3, "roi_gups_f32": ups / t / 1e9}
 
 
 if __name__ == "__main__":

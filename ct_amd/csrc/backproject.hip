@@ -44,6 +44,7 @@ struct BpArgs {
     float xpre, ypre, zpre, upre, vpre;
     float r2, fcd, uoff, ulo, uhi, vlo, vhi;
     int nproj, ntx, nty, nzb, cap, accumulate, vec4;
+    int zshift;              // z0 mod 16: runs start at grid z multiples of 16, zshift voxels before the slab
     int dma16;               // 8-byte entries staged with 16-byte DMA (H even: rows start even)
     int pitch_mod, pitch_rem;  // LDS column pitch = rows rounded up to pitch_rem mod pitch_mod
 };
@@ -51,14 +52,14 @@ struct BpArgs {
 // Entry formats: FP32 (16 B), FP16 and BF16 (8 B: {A,dU} {dV,dUV} as 16-bit pairs).
 struct EF32 {
     using T = float4;
-    static constexpr int kBytes = 16, kDma = 16;
+    static constexpr int kBytes = 16;
     __device__ static float eval(const T& e, float fu, float fv) {
         return fmaf(fv, fmaf(fu, e.w, e.z), fmaf(fu, e.y, e.x));
     }
 };
 struct EF16 {
     using T = uint2;
-    static constexpr int kBytes = 8, kDma = 4;
+    static constexpr int kBytes = 8;
     __device__ static float lo(uint32_t e) { return __half2float(__ushort_as_half((unsigned short)(e & 0xffffu))); }
     __device__ static float hi(uint32_t e) { return __half2float(__ushort_as_half((unsigned short)(e >> 16))); }
     __device__ static float eval(const T& e, float fu, float fv) {
@@ -67,7 +68,7 @@ struct EF16 {
 };
 struct EBF16 {
     using T = uint2;
-    static constexpr int kBytes = 8, kDma = 4;
+    static constexpr int kBytes = 8;
     __device__ static float lo(uint32_t e) { return __uint_as_float(e << 16); }
     __device__ static float hi(uint32_t e) { return __uint_as_float(e & 0xffff0000u); }
     __device__ static float eval(const T& e, float fu, float fv) {
@@ -104,7 +105,10 @@ __device__ inline int box_nc(int4 b) { return b.z & 0x07ffffff; }
 __device__ inline int box_nr(int4 b) { return b.w & 0xffff; }
 __device__ inline int box_pitch(int4 b) { return (unsigned)b.w >> 16; }
 
-template <int ZPL, class E>
+// DMA: bytes per lane of the global->LDS staging (16, or 4 for 8-byte entries).  A
+// compile-time width keeps the staging loop free of a run-time choice between the two
+// DMA forms, which measured markedly slower for the 8-byte entries.
+template <int ZPL, class E, int DMA>
 __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __restrict__ pp,
                                                       const float* __restrict__ scales,
                                                       const void* __restrict__ proj_raw,
@@ -127,8 +131,11 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
     const int lane = tid & 63, wave = tid >> 6;
     const int xi = tx * kTX + (tid & (kTX - 1));
     const int yi = ty * kTY + (tid / kTX);
+    // runs start at grid z indices that are multiples of 16 (the slab begins zshift
+    // voxels into the first run); run-local k is stored when zfirst <= k < zlast
     const int zb = zbi * ZPL;
-    const int zcount = min(ZPL, a.nzs - zb);
+    const int zfirst = a.zshift - zb;
+    const int zlast = a.nzs + a.zshift - zb;
     const bool colvol = xi < a.nx && yi < a.ny;
     const float x = (float)xi - a.xpre, y = (float)yi - a.ypre;
     const bool incyl = colvol && (x * x + y * y < a.r2);
@@ -137,10 +144,13 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
     const float cxb = (float)min(tx * kTX + kTX - 1, a.nx - 1) - a.xpre;
     const float cya = (float)(ty * kTY) - a.ypre;
     const float cyb = (float)min(ty * kTY + kTY - 1, a.ny - 1) - a.ypre;
-    // the z-run always spans ZPL voxels (those past the slab are computed, never stored),
-    // so boxes and the fast-path test cover the full run
-    const float zw0 = (float)(a.z0 + zb) - a.zpre;
+    // the z-run always spans ZPL voxels (those outside the slab are computed, never
+    // stored), so boxes and the fast-path test cover the full run.  zwa[j] is the world z
+    // of the run's j-th 16-voxel anchor (see det_row_index).
+    const int zrun = a.z0 - a.zshift + zb;
+    const float zw0 = (float)zrun - a.zpre;
     const float zw1 = zw0 + (float)(ZPL - 1);
+    const float zwa1 = (float)(zrun + 16) - a.zpre;
 
     float acc[ZPL];
 #pragma unroll
@@ -196,15 +206,14 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
         };
 
         // Stage the box of projection p into LDS buffer `buf`: wave w copies detector
-        // columns w, w+4, ...; a column's entries are contiguous in global memory and in LDS.
-        // Stage the box of projection p into LDS buffer `buf`: wave w copies detector
         // columns w, w+4, ...; a column's entries are contiguous in global memory and in
-        // LDS.  All addresses are wave-uniform (SGPR arithmetic); each lane moves kDma bytes.
+        // LDS.  All addresses are wave-uniform (SGPR arithmetic); each lane moves DMA bytes.
         const int wave_u = __builtin_amdgcn_readfirstlane(wave);
         auto stage = [&](int p, int4 bx, int buf) {
             const int us = bx.x, vs = bx.y, nr = box_nr(bx), nc = box_nc(bx), pitch = box_pitch(bx);
-            const bool wide = E::kBytes == 16 || a.dma16;
-            const int ub = wide ? 16 : 4;  // bytes per lane per DMA
+            static_assert(DMA == 16 || (DMA == 4 && E::kBytes == 8), "16-byte entries stage 16 bytes per lane");
+            constexpr bool wide = DMA == 16;
+            constexpr int ub = DMA;  // bytes per lane per DMA
             const int units = wide ? (nr * E::kBytes + 15) / 16 : nr * (E::kBytes / 4);
             const char* src = reinterpret_cast<const char*>(proj) +
                               (((size_t)p * a.W + us) * a.H + vs) * E::kBytes;
@@ -217,7 +226,7 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
                     if (lane < units - j) {
                         const void* g = gcol + (size_t)j * ub + lane_off;
                         void* l = (void*)(lcol + (size_t)j * ub);
-                        if (wide) lds_dma16(g, l);
+                        if constexpr (wide) lds_dma16(g, l);
                         else lds_dma4(g, l);
                     }
                 }
@@ -232,9 +241,7 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
             float t = (-1.0f) * x * g.x + y * g.y;
             t += a.uoff;
             const float s = x * g.y + y * g.x;
-            // v_rcp_f32 (1 ulp) instead of the IEEE division sequence; the reference's GPU
-            // kernel also divides with fast math
-            const float U = a.fcd * __builtin_amdgcn_rcpf(a.fcd - s);
+            const float U = cone_magnification(a.fcd, s);
             const float u = t * U;
             const bool cv = incyl && u >= a.ulo && u <= a.uhi;
             const float uIdx = u + a.upre;
@@ -253,21 +260,24 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
                     const float fue = cv ? fu : 0.0f;
                     const int base = cv ? (u0 - us) * nr - vs : -(int)floorf(a.vpre);
                     const T* tb = tile + base;
-                    const float vI0 = fmaf(-(zw0 + g.z), Ue, a.vpre);
+                    const float vA0 = det_row_anchor(zw0, g.z, Ue, a.vpre);
+                    const float vA1 = ZPL > 16 ? det_row_anchor(zwa1, g.z, Ue, a.vpre) : vA0;
 #pragma unroll
                     for (int k = 0; k < ZPL; ++k) {
-                        const float vI = fmaf(-(float)k, Ue, vI0);
+                        const float vI = det_row_index(k & 15, Ue, k < 16 ? vA0 : vA1);
                         const T e = tb[cvt_flr(vI)];
                         acc[k] = fmaf(we, E::eval(e, fue, __builtin_amdgcn_fractf(vI)), acc[k]);
                     }
                 } else {
                     const T* tb = tile + ((u0 - us) * nr - vs);
+                    const float vA0 = det_row_anchor(zw0, g.z, U, a.vpre);
+                    const float vA1 = ZPL > 16 ? det_row_anchor(zwa1, g.z, U, a.vpre) : vA0;
 #pragma unroll
                     for (int k = 0; k < ZPL; ++k) {
                         const float zk = zw0 + (float)k;
                         const float v = (zk + g.z) * U;
                         if (cv && v >= a.vlo && v <= a.vhi) {
-                            const float vI = (-1.0f) * v + a.vpre;
+                            const float vI = det_row_index(k & 15, U, k < 16 ? vA0 : vA1);
                             const int v0 = (int)vI;
                             const T e = tb[v0];
                             acc[k] = fmaf(w, E::eval(e, fu, vI - (float)v0), acc[k]);
@@ -276,12 +286,14 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
                 }
             } else {
                 const T* src = proj + ((size_t)p * a.W + u0) * a.H;
+                const float vA0 = det_row_anchor(zw0, g.z, U, a.vpre);
+                const float vA1 = ZPL > 16 ? det_row_anchor(zwa1, g.z, U, a.vpre) : vA0;
 #pragma unroll
                 for (int k = 0; k < ZPL; ++k) {
                     const float zk = zw0 + (float)k;
                     const float v = (zk + g.z) * U;
                     if (cv && v >= a.vlo && v <= a.vhi) {
-                        const float vI = (-1.0f) * v + a.vpre;
+                        const float vI = det_row_index(k & 15, U, k < 16 ? vA0 : vA1);
                         const int v0 = (int)vI;
                         const T e = src[v0];
                         acc[k] = fmaf(w, E::eval(e, fu, vI - (float)v0), acc[k]);
@@ -323,9 +335,9 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
     }
 
     if (colvol) {
-        float* dst = vol + ((size_t)xi * a.ny + yi) * a.nzs + zb;
-        if (a.vec4 && zcount == ZPL) {
-            float4* d4 = reinterpret_cast<float4*>(dst);
+        float* col = vol + ((size_t)xi * a.ny + yi) * a.nzs;
+        if (a.vec4 && zfirst <= 0 && zlast >= ZPL) {
+            float4* d4 = reinterpret_cast<float4*>(col + zb);
 #pragma unroll
             for (int k = 0; k < ZPL; k += 4) {
                 float4 v = make_float4(acc[k], acc[k + 1], acc[k + 2], acc[k + 3]);
@@ -338,7 +350,10 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
         } else {
 #pragma unroll
             for (int k = 0; k < ZPL; ++k) {
-                if (k < zcount) dst[k] = a.accumulate ? dst[k] + acc[k] : acc[k];
+                if (k >= zfirst && k < zlast) {
+                    float* d = col + (zb - a.zshift + k);
+                    *d = a.accumulate ? *d + acc[k] : acc[k];
+                }
             }
         }
     }
@@ -480,7 +495,7 @@ template <int ZPL>
 ctg_status launch_zpl(const BpLaunch& L, BpArgs a, hipStream_t st) {
     const ctg_geometry& g = *L.g;
     const int ebytes = L.store == CTG_STORE_F32 ? 16 : 8;
-    a.nzb = (L.nz + ZPL - 1) / ZPL;
+    a.nzb = (L.nz + a.zshift + ZPL - 1) / ZPL;
     // 8-byte entries are staged 4 bytes per lane by default: the 16-byte form needs an
     // even LDS pitch, which measured slower (more bank conflicts on the 8-byte reads) than
     // the odd pitch.  CTG_BP_DMA16=1 selects it (needs an even detector height).
@@ -497,7 +512,9 @@ ctg_status launch_zpl(const BpLaunch& L, BpArgs a, hipStream_t st) {
             a.pitch_rem = r % m;
         }
     }
-    int cap = L.h_params ? sample_max_box(g, L.h_params, L.nproj, L.z0, L.nz, ZPL, a.dma16, a.pitch_mod, a.pitch_rem) : 0;
+    int cap = L.h_params ? sample_max_box(g, L.h_params, L.nproj, L.z0 - a.zshift, L.nz + a.zshift, ZPL, a.dma16,
+                                          a.pitch_mod, a.pitch_rem)
+                         : 0;
     cap = cap + cap / 32 + 16;
     cap = std::min(cap, kMaxTileBytes / ebytes);
     cap &= ~7;  // keeps the second buffer and every column start 16-byte aligned
@@ -505,22 +522,25 @@ ctg_status launch_zpl(const BpLaunch& L, BpArgs a, hipStream_t st) {
     const size_t lds = (size_t)2 * cap * ebytes;
     const long long grid = (long long)a.ntx * a.nty * a.nzb;
     if (grid > 0x7fffffffll) return fail(CTG_ERROR_UNSUPPORTED, "backprojection grid too large");
+#define CTG_BP_LAUNCH(E, DMA)                                                                           \
+    hipLaunchKernelGGL((bp_kernel<ZPL, E, DMA>), dim3((unsigned)grid), dim3(kThreads), lds, st, a, L.d_params, \
+                       L.d_scales, L.d_filtered, L.d_volume)
     switch (L.store) {
         case CTG_STORE_F32:
-            hipLaunchKernelGGL((bp_kernel<ZPL, EF32>), dim3((unsigned)grid), dim3(kThreads), lds, st, a,
-                               L.d_params, L.d_scales, L.d_filtered, L.d_volume);
+            CTG_BP_LAUNCH(EF32, 16);
             break;
         case CTG_STORE_F16:
-            hipLaunchKernelGGL((bp_kernel<ZPL, EF16>), dim3((unsigned)grid), dim3(kThreads), lds, st, a,
-                               L.d_params, L.d_scales, L.d_filtered, L.d_volume);
+            if (a.dma16) CTG_BP_LAUNCH(EF16, 16);
+            else CTG_BP_LAUNCH(EF16, 4);
             break;
         case CTG_STORE_BF16:
-            hipLaunchKernelGGL((bp_kernel<ZPL, EBF16>), dim3((unsigned)grid), dim3(kThreads), lds, st, a,
-                               L.d_params, L.d_scales, L.d_filtered, L.d_volume);
+            if (a.dma16) CTG_BP_LAUNCH(EBF16, 16);
+            else CTG_BP_LAUNCH(EBF16, 4);
             break;
         default:
             return fail(CTG_ERROR_INVALID_ARGUMENT, "unknown store type %d", (int)L.store);
     }
+#undef CTG_BP_LAUNCH
     CTG_HIP(hipGetLastError());
     return CTG_OK;
 }
@@ -537,7 +557,9 @@ ctg_status launch_backproject(const BpLaunch& L, hipStream_t st) {
     BpArgs a = make_args(g, L.nproj, L.z0, L.nz, L.accumulate);
     a.ntx = (g.nx + kTX - 1) / kTX;
     a.nty = (g.ny + kTY - 1) / kTY;
-    a.vec4 = (L.nz % 4 == 0 && ((uintptr_t)L.d_volume % 16) == 0) ? 1 : 0;
+    a.zshift = L.z0 & 15;
+    // whole runs store 16-byte vectors when the slab starts on a run boundary
+    a.vec4 = (a.zshift == 0 && L.nz % 4 == 0 && ((uintptr_t)L.d_volume % 16) == 0) ? 1 : 0;
     // z per lane: 32 by default; CTG_BP_ZPL=16 selects the shorter run
     const char* env = std::getenv("CTG_BP_ZPL");
     const int zpl = (env && std::atoi(env) == 16) ? 16 : 32;

@@ -34,6 +34,25 @@ void clear_error();
         if (s_ != CTG_OK) return s_;        \
     } while (0)
 
+// Cone-beam magnification U = FCD / (FCD - s) of a voxel at depth s along the central
+// ray (Ct/cuda_ct.cu:250, Ct/CtVolume.cpp:1004).  v_rcp_f32 (1 ulp) replaces the IEEE
+// division sequence, as the reference's fast-math kernel did.  The backprojection and its
+// adjoint (the forward projector) both take U from here: identical U means identical
+// window masks and bilinear weights, so the pair stays an exact transpose.
+__device__ inline float cone_magnification(float fcd, float s) { return fcd * __builtin_amdgcn_rcpf(fcd - s); }
+
+// Detector row coordinate vIndex = vPre - (z + h) U of a voxel (Ct/CtVolume.cpp:1008,
+// 1455), evaluated from an anchor every 16 grid slices: for grid z index zi, with
+// za = zi rounded down to a multiple of 16 and zwa its world z,
+//     vIndex = fma(-(zi - za), U, fma(-(zwa + h), U, vPre)).
+// The anchor costs one FMA per (voxel column, projection, 16 slices), the voxel one FMA.
+// At |vIndex| ~ 500 one float ulp is 3e-5 pixel, so different evaluation orders place a
+// voxel differently by that much; the backprojection and the forward projector both
+// use exactly this expression, whatever their run lengths and slab boundaries, so their
+// bilinear weights agree bit for bit and each is the exact transpose of the other.
+__device__ inline float det_row_anchor(float zwa, float h, float U, float vpre) { return fmaf(-(zwa + h), U, vpre); }
+__device__ inline float det_row_index(int k, float U, float anchor) { return fmaf(-(float)k, U, anchor); }
+
 // Grow-only device buffer owned by a context.
 struct DevBuf {
     void* ptr = nullptr;

@@ -17,10 +17,12 @@ namespace ctg {
 namespace {
 
 constexpr int kTX = 16, kTY = 16, kThreads = 256, kZR = 16;
+static_assert(kZR == 16, "a forward-projection run is one 16-slice anchor block (det_row_index)");
 constexpr int kFpCap = 4096;  // LDS accumulation tile capacity (floats)
 
 struct FpArgs {
     int W, H, nx, ny, nzs, z0, nproj, ntx, nty;
+    int zshift;  // z0 mod 16: runs start at grid z multiples of 16
     float xpre, ypre, zpre, upre, vpre;
     float r2, fcd, uoff, ulo, uhi, vlo, vhi;
 };
@@ -35,16 +37,18 @@ __global__ __launch_bounds__(kThreads) void fp_kernel(FpArgs a, const float4* __
     const int tid = threadIdx.x;
     const int xi = tx * kTX + (tid & (kTX - 1));
     const int yi = ty * kTY + (tid / kTX);
-    const int zcount = min(kZR, a.nzs - zb);
+    // run-local k is a slab voxel when zfirst <= k < zlast (slab index zb - zshift + k)
+    const int zfirst = a.zshift - zb;
+    const int zlast = a.nzs + a.zshift - zb;
     const bool colvol = xi < a.nx && yi < a.ny;
     const float x = (float)xi - a.xpre, y = (float)yi - a.ypre;
     const bool incyl = colvol && (x * x + y * y < a.r2);
 
     float f[kZR];
     {
-        const float* src = vol + ((size_t)(colvol ? xi : 0) * a.ny + (colvol ? yi : 0)) * a.nzs + zb;
+        const float* col = vol + ((size_t)(colvol ? xi : 0) * a.ny + (colvol ? yi : 0)) * a.nzs;
 #pragma unroll
-        for (int k = 0; k < kZR; ++k) f[k] = (incyl && k < zcount) ? src[k] : 0.0f;
+        for (int k = 0; k < kZR; ++k) f[k] = (incyl && k >= zfirst && k < zlast) ? col[zb - a.zshift + k] : 0.0f;
     }
     const float cxa = (float)(tx * kTX) - a.xpre;
     const float cxb = (float)min(tx * kTX + kTX - 1, a.nx - 1) - a.xpre;
@@ -53,8 +57,9 @@ __global__ __launch_bounds__(kThreads) void fp_kernel(FpArgs a, const float4* __
     const float ndx = cxa > 0.0f ? cxa : (cxb < 0.0f ? cxb : 0.0f);
     const float ndy = cya > 0.0f ? cya : (cyb < 0.0f ? cyb : 0.0f);
     if (!(ndx * ndx + ndy * ndy < a.r2)) return;  // uniform: no voxel of the tile in the cylinder
-    const float zw0 = (float)(a.z0 + zb) - a.zpre;
-    const float zw1 = zw0 + (float)(zcount - 1);
+    // the run is one 16-slice anchor block of the grid (det_row_index)
+    const float zw0 = (float)(a.z0 - a.zshift + zb) - a.zpre;
+    const float zw1 = zw0 + (float)(kZR - 1);
 
     for (int p = 0; p < a.nproj; ++p) {
         const float4 g = pp[p];
@@ -97,7 +102,7 @@ __global__ __launch_bounds__(kThreads) void fp_kernel(FpArgs a, const float4* __
         float t = (-1.0f) * x * g.x + y * g.y;
         t += a.uoff;
         const float s = x * g.y + y * g.x;
-        const float U = a.fcd / (a.fcd - s);
+        const float U = cone_magnification(a.fcd, s);
         const float u = t * U;
         const bool cv = incyl && u >= a.ulo && u <= a.uhi;
         if (cv) {
@@ -105,12 +110,13 @@ __global__ __launch_bounds__(kThreads) void fp_kernel(FpArgs a, const float4* __
             const int u0 = (int)uIdx;
             const float fu = uIdx - (float)u0;
             const float w = U * U;
+            const float vA = det_row_anchor(zw0, g.z, U, a.vpre);
 #pragma unroll
             for (int k = 0; k < kZR; ++k) {
                 const float zk = zw0 + (float)k;
                 const float v = (zk + g.z) * U;
-                if (k < zcount && v >= a.vlo && v <= a.vhi) {
-                    const float vI = (-1.0f) * v + a.vpre;
+                if (k >= zfirst && k < zlast && v >= a.vlo && v <= a.vhi) {
+                    const float vI = det_row_index(k, U, vA);
                     const int v0 = (int)vI;
                     const float fv = vI - (float)v0;
                     const float wf = w * f[k];
@@ -278,11 +284,12 @@ ctg_status launch_forward_project(const ctg_geometry& g, const float4* d_params,
     if (nz <= 0 || nproj <= 0) return CTG_OK;
     FpArgs a{};
     a.W = g.det_width; a.H = g.det_height; a.nx = g.nx; a.ny = g.ny; a.nzs = nz; a.z0 = z0; a.nproj = nproj;
+    a.zshift = z0 & 15;
     a.ntx = (g.nx + kTX - 1) / kTX; a.nty = (g.ny + kTY - 1) / kTY;
     a.xpre = g.x_pre; a.ypre = g.y_pre; a.zpre = g.z_pre; a.upre = g.u_pre; a.vpre = g.v_pre;
     a.r2 = g.radius * g.radius; a.fcd = g.fcd; a.uoff = g.u_offset;
     a.ulo = g.u_lo; a.uhi = g.u_hi; a.vlo = g.v_lo; a.vhi = g.v_hi;
-    dim3 grid((unsigned)(a.ntx * a.nty), (unsigned)((nz + kZR - 1) / kZR));
+    dim3 grid((unsigned)(a.ntx * a.nty), (unsigned)((nz + a.zshift + kZR - 1) / kZR));
     hipLaunchKernelGGL(fp_kernel, grid, dim3(kThreads), 0, st, a, d_params, d_volume, d_proj);
     CTG_HIP(hipGetLastError());
     return CTG_OK;

@@ -186,7 +186,8 @@ ctg_status ctg_forward_project(ctg_context* ctx, const float* d_volume, int32_t 
 
 /* Backward of the linear FDK operator F = B . H . Wf (B backprojection, H row
  * filter, Wf Feldkamp weight) with respect to line integrals:
- * d_grad_lineint = Wf . H . B^T (d_grad_volume).  Uses d_work (nproj x W x H floats).
+ * d_grad_lineint = Wf . H . B^T (d_grad_volume), nproj row-major W x H images
+ * (overwritten; the filter and weight run in place there, no other workspace).
  * No counterpart in the reference. */
 ctg_status ctg_fdk_backward(ctg_context* ctx, const float* d_grad_volume, int32_t z0, int32_t nz,
                             int32_t p0, int32_t nproj, ctg_filter filter, float* d_grad_lineint,

@@ -96,3 +96,15 @@ def test_no_device_is_reported_not_faked():
 
 def test_version(ctg_lib):
     assert b"gfx950" in ctg_lib.ctg_version()
+
+
+def test_autograd_harness_has_no_cpu_path():
+    """The torch.autograd operators refuse to run without a HIP device."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    import ct_amd
+    from ct_amd.autograd import ConeBeamOperator
+    g = ct_amd.geometry_grid(32, 32, 160.0, 0.0, 16, 16, 16)
+    with pytest.raises(RuntimeError, match="no CPU path"):
+        ConeBeamOperator(g, [0.0, 90.0])
