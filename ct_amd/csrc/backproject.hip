@@ -44,9 +44,9 @@ struct BpArgs {
     float xpre, ypre, zpre, upre, vpre;
     float r2, fcd, uoff, ulo, uhi, vlo, vhi;
     int nproj, ntx, nty, nzb, cap, accumulate, vec4;
-    int zshift;              // z0 mod 16: runs start at grid z multiples of 16, zshift voxels before the slab
+    int zshift;              // z0 mod ZPL: runs start at grid z multiples of ZPL, zshift voxels before the slab
     int dma16;               // 8-byte entries staged with 16-byte DMA (H even: rows start even)
-    int pitch_mod, pitch_rem;  // LDS column pitch = rows rounded up to pitch_rem mod pitch_mod
+    int pitch_mod, pitch_rem;  // LDS column pitch = rows rounded up to pitch_rem mod pitch_mod (a power of two)
 };
 
 // Entry formats: FP32 (16 B), FP16 and BF16 (8 B: {A,dU} {dV,dUV} as 16-bit pairs).
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
     const int lane = tid & 63, wave = tid >> 6;
     const int xi = tx * kTX + (tid & (kTX - 1));
     const int yi = ty * kTY + (tid / kTX);
-    // runs start at grid z indices that are multiples of 16 (the slab begins zshift
+    // runs start at grid z indices that are multiples of ZPL (the slab begins zshift
     // voxels into the first run); run-local k is stored when zfirst <= k < zlast
     const int zb = zbi * ZPL;
     const int zfirst = a.zshift - zb;
@@ -145,12 +145,14 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
     const float cya = (float)(ty * kTY) - a.ypre;
     const float cyb = (float)min(ty * kTY + kTY - 1, a.ny - 1) - a.ypre;
     // the z-run always spans ZPL voxels (those outside the slab are computed, never
-    // stored), so boxes and the fast-path test cover the full run.  zwa[j] is the world z
-    // of the run's j-th 16-voxel anchor (see det_row_index).
+    // stored), so boxes and the fast-path test cover the full run.  Detector rows are
+    // taken from the run's 32-slice anchor (det_row_index): k + koff slices past it.
+    static_assert(ZPL == kRowAnchor || ZPL == kRowAnchor / 2, "a run lies inside one row anchor block");
     const int zrun = a.z0 - a.zshift + zb;
     const float zw0 = (float)zrun - a.zpre;
     const float zw1 = zw0 + (float)(ZPL - 1);
-    const float zwa1 = (float)(zrun + 16) - a.zpre;
+    const int koff = ZPL == kRowAnchor ? 0 : (zrun & (kRowAnchor - 1));
+    const float zwa = (float)(zrun - koff) - a.zpre;
 
     float acc[ZPL];
 #pragma unroll
@@ -195,9 +197,10 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
             const int ve = min((int)floorf(vmax) + 1, a.H - 1);
             const int nc = ue - us + 1, nr = ve - vs + 1;
             if (nc <= 0 || nr <= 0) { b.z = 0; return b; }
-            // LDS column pitch: rows rounded up to pitch_rem mod pitch_mod, so that
+            // LDS column pitch: rows rounded up to pitch_rem mod pitch_mod (a power of two,
+            // so a mask: a division here would raise the kernel's register peak), so that
             // neighbouring detector columns start in different LDS bank groups
-            const int pitch = nr + ((a.pitch_rem - nr) % a.pitch_mod + a.pitch_mod) % a.pitch_mod;
+            const int pitch = nr + ((a.pitch_rem - nr) & (a.pitch_mod - 1));
             b.x = us; b.y = vs; b.w = nr | (pitch << 16);
             // the fused vIndex of the fast path may round a hair below 0 at the top edge
             const int fast = vs0 >= 0 ? 1 : 0;
@@ -260,24 +263,22 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
                     const float fue = cv ? fu : 0.0f;
                     const int base = cv ? (u0 - us) * nr - vs : -(int)floorf(a.vpre);
                     const T* tb = tile + base;
-                    const float vA0 = det_row_anchor(zw0, g.z, Ue, a.vpre);
-                    const float vA1 = ZPL > 16 ? det_row_anchor(zwa1, g.z, Ue, a.vpre) : vA0;
+                    const float vA = det_row_anchor(zwa, g.z, Ue, a.vpre);
 #pragma unroll
                     for (int k = 0; k < ZPL; ++k) {
-                        const float vI = det_row_index(k & 15, Ue, k < 16 ? vA0 : vA1);
+                        const float vI = det_row_index((float)(k + koff), Ue, vA);
                         const T e = tb[cvt_flr(vI)];
                         acc[k] = fmaf(we, E::eval(e, fue, __builtin_amdgcn_fractf(vI)), acc[k]);
                     }
                 } else {
                     const T* tb = tile + ((u0 - us) * nr - vs);
-                    const float vA0 = det_row_anchor(zw0, g.z, U, a.vpre);
-                    const float vA1 = ZPL > 16 ? det_row_anchor(zwa1, g.z, U, a.vpre) : vA0;
+                    const float vA = det_row_anchor(zwa, g.z, U, a.vpre);
 #pragma unroll
                     for (int k = 0; k < ZPL; ++k) {
                         const float zk = zw0 + (float)k;
                         const float v = (zk + g.z) * U;
                         if (cv && v >= a.vlo && v <= a.vhi) {
-                            const float vI = det_row_index(k & 15, U, k < 16 ? vA0 : vA1);
+                            const float vI = det_row_index((float)(k + koff), U, vA);
                             const int v0 = (int)vI;
                             const T e = tb[v0];
                             acc[k] = fmaf(w, E::eval(e, fu, vI - (float)v0), acc[k]);
@@ -286,14 +287,13 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
                 }
             } else {
                 const T* src = proj + ((size_t)p * a.W + u0) * a.H;
-                const float vA0 = det_row_anchor(zw0, g.z, U, a.vpre);
-                const float vA1 = ZPL > 16 ? det_row_anchor(zwa1, g.z, U, a.vpre) : vA0;
+                const float vA = det_row_anchor(zwa, g.z, U, a.vpre);
 #pragma unroll
                 for (int k = 0; k < ZPL; ++k) {
                     const float zk = zw0 + (float)k;
                     const float v = (zk + g.z) * U;
                     if (cv && v >= a.vlo && v <= a.vhi) {
-                        const float vI = det_row_index(k & 15, U, k < 16 ? vA0 : vA1);
+                        const float vI = det_row_index((float)(k + koff), U, vA);
                         const int v0 = (int)vI;
                         const T e = src[v0];
                         acc[k] = fmaf(w, E::eval(e, fu, vI - (float)v0), acc[k]);
@@ -481,7 +481,7 @@ int sample_max_box(const ctg_geometry& g, const float4* hp, int nproj, int z0, i
                     const int ve = std::min((int)std::floor(vmax) + 1, g.det_height - 1);
                     if (ue >= us && ve >= vs) {
                         const int nr = ve - vs + 1;
-                        const int pitch = nr + ((prem - nr) % pmod + pmod) % pmod;
+                        const int pitch = nr + ((prem - nr) & (pmod - 1));
                         best = std::max(best, (ue - us + 1) * pitch);
                     }
                 }
@@ -495,6 +495,9 @@ template <int ZPL>
 ctg_status launch_zpl(const BpLaunch& L, BpArgs a, hipStream_t st) {
     const ctg_geometry& g = *L.g;
     const int ebytes = L.store == CTG_STORE_F32 ? 16 : 8;
+    a.zshift = L.z0 & (ZPL - 1);
+    // whole runs store 16-byte vectors when the slab starts on a run boundary
+    a.vec4 = (a.zshift == 0 && L.nz % 4 == 0 && ((uintptr_t)L.d_volume % 16) == 0) ? 1 : 0;
     a.nzb = (L.nz + a.zshift + ZPL - 1) / ZPL;
     // 8-byte entries are staged 4 bytes per lane by default: the 16-byte form needs an
     // even LDS pitch, which measured slower (more bank conflicts on the 8-byte reads) than
@@ -507,7 +510,8 @@ ctg_status launch_zpl(const BpLaunch& L, BpArgs a, hipStream_t st) {
     a.pitch_rem = a.dma16 ? 2 : 1;
     if (const char* env = std::getenv("CTG_BP_PITCH")) {
         int m = 0, r = 0;
-        if (std::sscanf(env, "%d,%d", &m, &r) == 2 && m >= 1 && r >= 0 && (!a.dma16 || (m % 2 == 0 && r % 2 == 0))) {
+        if (std::sscanf(env, "%d,%d", &m, &r) == 2 && m >= 1 && (m & (m - 1)) == 0 && r >= 0 &&
+            (!a.dma16 || (m % 2 == 0 && r % 2 == 0))) {
             a.pitch_mod = m;
             a.pitch_rem = r % m;
         }
@@ -557,9 +561,6 @@ ctg_status launch_backproject(const BpLaunch& L, hipStream_t st) {
     BpArgs a = make_args(g, L.nproj, L.z0, L.nz, L.accumulate);
     a.ntx = (g.nx + kTX - 1) / kTX;
     a.nty = (g.ny + kTY - 1) / kTY;
-    a.zshift = L.z0 & 15;
-    // whole runs store 16-byte vectors when the slab starts on a run boundary
-    a.vec4 = (a.zshift == 0 && L.nz % 4 == 0 && ((uintptr_t)L.d_volume % 16) == 0) ? 1 : 0;
     // z per lane: 32 by default; CTG_BP_ZPL=16 selects the shorter run
     const char* env = std::getenv("CTG_BP_ZPL");
     const int zpl = (env && std::atoi(env) == 16) ? 16 : 32;

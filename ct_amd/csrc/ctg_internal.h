@@ -42,16 +42,17 @@ void clear_error();
 __device__ inline float cone_magnification(float fcd, float s) { return fcd * __builtin_amdgcn_rcpf(fcd - s); }
 
 // Detector row coordinate vIndex = vPre - (z + h) U of a voxel (Ct/CtVolume.cpp:1008,
-// 1455), evaluated from an anchor every 16 grid slices: for grid z index zi, with
-// za = zi rounded down to a multiple of 16 and zwa its world z,
+// 1455), evaluated from an anchor every 32 grid slices: for grid z index zi, with
+// za = zi rounded down to a multiple of 32 and zwa its world z,
 //     vIndex = fma(-(zi - za), U, fma(-(zwa + h), U, vPre)).
-// The anchor costs one FMA per (voxel column, projection, 16 slices), the voxel one FMA.
+// The anchor costs one FMA per (voxel column, projection, 32 slices), the voxel one FMA.
 // At |vIndex| ~ 500 one float ulp is 3e-5 pixel, so different evaluation orders place a
 // voxel differently by that much; the backprojection and the forward projector both
 // use exactly this expression, whatever their run lengths and slab boundaries, so their
 // bilinear weights agree bit for bit and each is the exact transpose of the other.
+constexpr int kRowAnchor = 32;
 __device__ inline float det_row_anchor(float zwa, float h, float U, float vpre) { return fmaf(-(zwa + h), U, vpre); }
-__device__ inline float det_row_index(int k, float U, float anchor) { return fmaf(-(float)k, U, anchor); }
+__device__ inline float det_row_index(float k, float U, float anchor) { return fmaf(-k, U, anchor); }
 
 // Grow-only device buffer owned by a context.
 struct DevBuf {

@@ -17,7 +17,7 @@ namespace ctg {
 namespace {
 
 constexpr int kTX = 16, kTY = 16, kThreads = 256, kZR = 16;
-static_assert(kZR == 16, "a forward-projection run is one 16-slice anchor block (det_row_index)");
+static_assert(kRowAnchor % kZR == 0, "a forward-projection run lies inside one row anchor block (det_row_index)");
 constexpr int kFpCap = 4096;  // LDS accumulation tile capacity (floats)
 
 struct FpArgs {
@@ -57,9 +57,12 @@ __global__ __launch_bounds__(kThreads) void fp_kernel(FpArgs a, const float4* __
     const float ndx = cxa > 0.0f ? cxa : (cxb < 0.0f ? cxb : 0.0f);
     const float ndy = cya > 0.0f ? cya : (cyb < 0.0f ? cyb : 0.0f);
     if (!(ndx * ndx + ndy * ndy < a.r2)) return;  // uniform: no voxel of the tile in the cylinder
-    // the run is one 16-slice anchor block of the grid (det_row_index)
-    const float zw0 = (float)(a.z0 - a.zshift + zb) - a.zpre;
+    // detector rows come from the run's 32-slice anchor (det_row_index): k + koff past it
+    const int zrun = a.z0 - a.zshift + zb;
+    const float zw0 = (float)zrun - a.zpre;
     const float zw1 = zw0 + (float)(kZR - 1);
+    const int koff = zrun & (kRowAnchor - 1);
+    const float zwa = (float)(zrun - koff) - a.zpre;
 
     for (int p = 0; p < a.nproj; ++p) {
         const float4 g = pp[p];
@@ -110,13 +113,13 @@ __global__ __launch_bounds__(kThreads) void fp_kernel(FpArgs a, const float4* __
             const int u0 = (int)uIdx;
             const float fu = uIdx - (float)u0;
             const float w = U * U;
-            const float vA = det_row_anchor(zw0, g.z, U, a.vpre);
+            const float vA = det_row_anchor(zwa, g.z, U, a.vpre);
 #pragma unroll
             for (int k = 0; k < kZR; ++k) {
                 const float zk = zw0 + (float)k;
                 const float v = (zk + g.z) * U;
                 if (k >= zfirst && k < zlast && v >= a.vlo && v <= a.vhi) {
-                    const float vI = det_row_index(k, U, vA);
+                    const float vI = det_row_index((float)(k + koff), U, vA);
                     const int v0 = (int)vI;
                     const float fv = vI - (float)v0;
                     const float wf = w * f[k];

@@ -41,8 +41,8 @@ def _dot(a, b):
 
 def _sub_geometry(g, xi, yi, z0, nz):
     """Geometry of the 1 x 1 x nz column of grid g starting at voxel (xi, yi, z0): the
-    same world coordinates, so the same per-voxel arithmetic when z0 is a multiple of 16
-    (the projectors evaluate detector rows from 16-slice anchors of the grid)."""
+    same world coordinates, so the same per-voxel arithmetic when z0 is a multiple of 32
+    (the projectors evaluate detector rows from 32-slice anchors of the grid)."""
     import ct_amd
     h = ct_amd.Geometry()
     for name, _ in ct_amd.Geometry._fields_:
@@ -112,7 +112,7 @@ def test_gradient_finite_differences_and_cpu_fp64(full, oracle):
 
     The loss at x +- h e_i is evaluated through the forward projector using A's
     linearity: A(x +- h e_i) = A x +- h A e_i, where A e_i is the same GPU projector
-    run on the 16-slice column block of the grid that holds voxel i (identical world
+    run on the 32-slice column block of the grid that holds voxel i (identical world
     coordinates and anchors, so identical arithmetic).  Differences are accumulated in
     FP64 over every detector pixel.
 
@@ -134,9 +134,9 @@ def test_gradient_finite_differences_and_cpu_fp64(full, oracle):
     h = 1e-3
     fd, ag = [], []
     for (xi, yi, zi) in voxels:
-        za = zi - zi % 16
-        one = ConeBeamOperator(_sub_geometry(g, xi, yi, za, 16), angles)
-        e = torch.zeros((1, 1, 16), dtype=torch.float32, device=full["dev"])
+        za = zi - zi % 32
+        one = ConeBeamOperator(_sub_geometry(g, xi, yi, za, 32), angles)
+        e = torch.zeros((1, 1, 32), dtype=torch.float32, device=full["dev"])
         e[0, 0, zi - za] = 1.0
         a = one.project(e).double()
         lp = 0.5 * torch.sum((r + h * a) ** 2)
