@@ -16,9 +16,12 @@
 //  - a workgroup (256 lanes = 16x16 columns) stages, per projection, the bounding box of
 //    its voxel block's detector footprint into LDS with global->LDS DMA
 //    (global_load_lds), one detector column per DMA burst, double-buffered so that the
-//    DMA of projection p+1 runs under the compute of p;
+//    DMA of projection p+1 runs under the compute of p (the two buffers reach the
+//    compute as __restrict__ pointers, without which the compiler waits for the DMA
+//    before the first LDS read);
 //  - per z the lane does one FMA for vIndex, v_cvt_flr/v_fract, one address op, the LDS
-//    read and 4 FMAs; the per-column geometry (one division) is amortised over the run;
+//    read and 4 FMAs, with the reads software-pipelined in small groups; the
+//    per-column geometry (one reciprocal) is amortised over the run;
 //  - boxes are computed by wave 0 for 64 projections at a time (one lane per projection);
 //    a projection whose box exceeds the tile capacity reads entries from global memory
 //    instead, so any geometry stays correct;
@@ -53,6 +56,8 @@ struct BpArgs {
 struct EF32 {
     using T = float4;
     static constexpr int kBytes = 16;
+    static constexpr int kMinWaves = 1;  // LDS limits FP32 tiles to 3 workgroups per CU anyway
+    static constexpr int kGrp = 4;       // LDS reads per software-pipeline group
     __device__ static float eval(const T& e, float fu, float fv) {
         return fmaf(fv, fmaf(fu, e.w, e.z), fmaf(fu, e.y, e.x));
     }
@@ -60,6 +65,8 @@ struct EF32 {
 struct EF16 {
     using T = uint2;
     static constexpr int kBytes = 8;
+    static constexpr int kMinWaves = 4;  // VALU-bound: keep 4 waves per SIMD (at most 128 VGPRs)
+    static constexpr int kGrp = 2;       // (groups of 4 need spills at 128 VGPRs: measured slower)
     __device__ static float lo(uint32_t e) { return __half2float(__ushort_as_half((unsigned short)(e & 0xffffu))); }
     __device__ static float hi(uint32_t e) { return __half2float(__ushort_as_half((unsigned short)(e >> 16))); }
     __device__ static float eval(const T& e, float fu, float fv) {
@@ -69,6 +76,8 @@ struct EF16 {
 struct EBF16 {
     using T = uint2;
     static constexpr int kBytes = 8;
+    static constexpr int kMinWaves = 4;
+    static constexpr int kGrp = 2;
     __device__ static float lo(uint32_t e) { return __uint_as_float(e << 16); }
     __device__ static float hi(uint32_t e) { return __uint_as_float(e & 0xffff0000u); }
     __device__ static float eval(const T& e, float fu, float fv) {
@@ -109,7 +118,7 @@ __device__ inline int box_pitch(int4 b) { return (unsigned)b.w >> 16; }
 // compile-time width keeps the staging loop free of a run-time choice between the two
 // DMA forms, which measured markedly slower for the 8-byte entries.
 template <int ZPL, class E, int DMA>
-__global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __restrict__ pp,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E::kMinWaves))) void bp_kernel(BpArgs a, const float4* __restrict__ pp,
                                                       const float* __restrict__ scales,
                                                       const void* __restrict__ proj_raw,
                                                       float* __restrict__ vol) {
@@ -212,7 +221,7 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
         // columns w, w+4, ...; a column's entries are contiguous in global memory and in
         // LDS.  All addresses are wave-uniform (SGPR arithmetic); each lane moves DMA bytes.
         const int wave_u = __builtin_amdgcn_readfirstlane(wave);
-        auto stage = [&](int p, int4 bx, int buf) {
+        auto stage = [&](unsigned char* dst, int p, int4 bx) {
             const int us = bx.x, vs = bx.y, nr = box_nr(bx), nc = box_nc(bx), pitch = box_pitch(bx);
             static_assert(DMA == 16 || (DMA == 4 && E::kBytes == 8), "16-byte entries stage 16 bytes per lane");
             constexpr bool wide = DMA == 16;
@@ -220,7 +229,6 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
             const int units = wide ? (nr * E::kBytes + 15) / 16 : nr * (E::kBytes / 4);
             const char* src = reinterpret_cast<const char*>(proj) +
                               (((size_t)p * a.W + us) * a.H + vs) * E::kBytes;
-            unsigned char* dst = smem + (size_t)buf * a.cap * E::kBytes;
             const unsigned lane_off = (unsigned)lane * ub;
             for (int c = wave_u; c < nc; c += kWaves) {
                 const char* gcol = src + (size_t)c * a.H * E::kBytes;
@@ -236,11 +244,9 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
             }
         };
 
-        auto compute = [&](int p, int4 bx, int buf) {
+        auto compute = [&](const T* tile, int p, int4 bx, float4 g, float sc) {
             const int kind = box_kind(bx);
             if (kind == 0) return;
-            const float4 g = pp[p];
-            const float sc = scales ? scales[p] : 1.0f;
             float t = (-1.0f) * x * g.x + y * g.y;
             t += a.uoff;
             const float s = x * g.y + y * g.x;
@@ -252,7 +258,6 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
             const float fu = uIdx - (float)u0;
             const float w = U * U * sc;
             if (kind == 1) {
-                const T* tile = reinterpret_cast<const T*>(smem + (size_t)buf * a.cap * E::kBytes);
                 const int us = bx.x, vs = bx.y, nr = box_pitch(bx);
                 const bool lane_full = !cv || ((zw0 + g.z) * U >= a.vlo && (zw1 + g.z) * U <= a.vhi);
                 if (box_fast(bx) && __all(lane_full)) {
@@ -264,11 +269,30 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
                     const int base = cv ? (u0 - us) * nr - vs : -(int)floorf(a.vpre);
                     const T* tb = tile + base;
                     const float vA = det_row_anchor(zwa, g.z, Ue, a.vpre);
+                    // software pipeline in groups of kGrp: the LDS reads of group g+1 are
+                    // issued before the arithmetic of group g, and scheduling barriers
+                    // keep the compiler from trading those in-flight reads for hoisted
+                    // row-index arithmetic
+                    constexpr int kGrp = E::kGrp;
+                    T e[2][kGrp];
+                    float fv[2][kGrp];
+                    auto fetch = [&](int g0, int slot) {
 #pragma unroll
-                    for (int k = 0; k < ZPL; ++k) {
-                        const float vI = det_row_index((float)(k + koff), Ue, vA);
-                        const T e = tb[cvt_flr(vI)];
-                        acc[k] = fmaf(we, E::eval(e, fue, __builtin_amdgcn_fractf(vI)), acc[k]);
+                        for (int j = 0; j < kGrp; ++j) {
+                            const float vI = det_row_index((float)(g0 + j + koff), Ue, vA);
+                            e[slot][j] = tb[cvt_flr(vI)];
+                            fv[slot][j] = __builtin_amdgcn_fractf(vI);
+                        }
+                    };
+                    fetch(0, 0);
+#pragma unroll
+                    for (int g0 = 0; g0 < ZPL; g0 += kGrp) {
+                        const int slot = (g0 / kGrp) & 1;
+                        if (g0 + kGrp < ZPL) fetch(g0 + kGrp, slot ^ 1);
+#pragma unroll
+                        for (int j = 0; j < kGrp; ++j)
+                            acc[g0 + j] = fmaf(we, E::eval(e[slot][j], fue, fv[slot][j]), acc[g0 + j]);
+                        __builtin_amdgcn_sched_barrier(0);
                     }
                 } else {
                     const T* tb = tile + ((u0 - us) * nr - vs);
@@ -307,16 +331,31 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
         // batch ahead by wave 0).
         if (tid < 64) boxes[0][tid] = tid < a.nproj ? box_for(tid) : make_int4(0, 0, 0, 0);
         __syncthreads();
+        auto buffer = [&](int p) { return smem + (size_t)(p & 1) * a.cap * E::kBytes; };
         if (a.nproj > 0) {
             const int4 b0 = boxes[0][0];
-            if (box_kind(b0) == 1) stage(0, b0, 0);
+            if (box_kind(b0) == 1) stage(buffer(0), 0, b0);
         }
+        // One projection step: stage projection p+1 into `nxt` (LDS DMA) and accumulate
+        // projection p from `cur`.  The buffers are disjoint; declaring them __restrict__
+        // lets the compiler issue the LDS reads of p while the DMA of p+1 is in flight
+        // (otherwise it waits for all outstanding DMA before the first read).
+        auto step = [&](const T* __restrict__ cur, unsigned char* __restrict__ nxt, int p, int4 bx, int4 bn,
+                        float4 g, float sc) {
+            if (box_kind(bn) == 1) stage(nxt, p + 1, bn);
+            compute(cur, p, bx, g, sc);
+        };
         auto uniform_box = [](int4 b) {
             return make_int4(__builtin_amdgcn_readfirstlane(b.x), __builtin_amdgcn_readfirstlane(b.y),
                              __builtin_amdgcn_readfirstlane(b.z), __builtin_amdgcn_readfirstlane(b.w));
         };
         for (int p = 0; p < a.nproj; ++p) {
             const int4 bx = uniform_box(boxes[(p >> 6) & 1][p & 63]);
+            // projection parameters are loaded before the wait below, which covers them:
+            // a global load issued after the DMA of p+1 would make its first use wait for
+            // that DMA too (the memory counter retires in order)
+            const float4 g = pp[p];
+            const float sc = scales ? scales[p] : 1.0f;
             // this wave's DMA for projection p has landed; the barrier publishes every
             // wave's and guarantees that nobody still reads buffer (p+1)&1 or the box
             // batch about to be overwritten
@@ -326,11 +365,9 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BpArgs a, const float4* __
                 const int pn = p + 64 + tid;
                 boxes[((p >> 6) + 1) & 1][tid] = pn < a.nproj ? box_for(pn) : make_int4(0, 0, 0, 0);
             }
-            if (p + 1 < a.nproj) {
-                const int4 bn = uniform_box(boxes[((p + 1) >> 6) & 1][(p + 1) & 63]);
-                if (box_kind(bn) == 1) stage(p + 1, bn, (p + 1) & 1);
-            }
-            compute(p, bx, p & 1);
+            const int4 bn = p + 1 < a.nproj ? uniform_box(boxes[((p + 1) >> 6) & 1][(p + 1) & 63])
+                                            : make_int4(0, 0, 0, 0);
+            step(reinterpret_cast<const T*>(buffer(p)), buffer(p + 1), p, bx, bn, g, sc);
         }
     }
 

@@ -93,3 +93,52 @@ def shepp_logan_3d(unit: float, scale: float = 1.0):
 def circular_angles(nproj: int):
     """Equiangular full scan, 360*i/P degrees (ccw, as Ct stores them)."""
     return [360.0 * i / nproj for i in range(nproj)]
+
+
+def _rotation(phi, theta, psi):
+    """R = Rz(phi) Ry(theta) Rz(psi); its columns are the ellipsoid's axes."""
+    c1, s1, c2, s2, c3, s3 = (math.cos(phi), math.sin(phi), math.cos(theta), math.sin(theta),
+                              math.cos(psi), math.sin(psi))
+    return [[c1 * c2 * c3 - s1 * s3, -c1 * c2 * s3 - s1 * c3, c1 * s2],
+            [s1 * c2 * c3 + c1 * s3, -s1 * c2 * s3 + c1 * c3, s1 * s2],
+            [-s2 * c3, s2 * s3, c2]]
+
+
+def needles(n: int, seed: int, unit: float, mu: float, radius: float = 2.0, half_length: float = 0.6):
+    """Thin high-contrast rods: ellipsoids with two semi-axes of `radius` voxels and a
+    long semi-axis of half_length * unit, at random centres and orientations."""
+    rng = Xoshiro256ss(seed)
+    out = []
+    for _ in range(n):
+        c = [rng.uniform(-0.4, 0.4) * unit for _ in range(3)]
+        ang = [rng.uniform(0, 2 * math.pi), rng.uniform(0, math.pi), rng.uniform(0, 2 * math.pi)]
+        out.append(c + [radius, radius, half_length * unit] + ang + [mu])
+    return out
+
+
+def voxelize(ellipsoids, geometry, device, dtype=None):
+    """Attenuation of the ellipsoid phantom sampled at the voxels of `geometry`
+    (voxel (i, j, k) at world (i - x_pre, j - y_pre, k - z_pre), the backprojection's
+    model), as an (nx, ny, nz) tensor on `device`.  Used to score reconstructions."""
+    import torch
+    dtype = dtype or torch.float32
+    g = geometry
+    out = torch.zeros((g.nx, g.ny, g.nz), dtype=dtype, device=device)
+    pre = (g.x_pre, g.y_pre, g.z_pre)
+    size = (g.nx, g.ny, g.nz)
+    for e in ellipsoids:
+        c, ax, (phi, theta, psi), mu = e[0:3], e[3:6], e[6:9], e[9]
+        R = _rotation(phi, theta, psi)
+        reach = max(ax) + 1.0
+        lo = [max(0, int(math.floor(c[d] + pre[d] - reach))) for d in range(3)]
+        hi = [min(size[d], int(math.ceil(c[d] + pre[d] + reach)) + 1) for d in range(3)]
+        if any(h <= l for l, h in zip(lo, hi)):
+            continue
+        coords = [torch.arange(lo[d], hi[d], device=device, dtype=torch.float64) - pre[d] - c[d] for d in range(3)]
+        X, Y, Z = torch.meshgrid(*coords, indexing="ij")
+        q = 0.0
+        for a in range(3):  # body coordinate a = (R^T (p - c))_a / ax_a
+            qa = (R[0][a] * X + R[1][a] * Y + R[2][a] * Z) / ax[a]
+            q = q + qa * qa
+        out[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] += (q < 1.0).to(dtype) * mu
+    return out
