@@ -90,15 +90,21 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--det", type=int, default=1024)
-    ap.add_argument("--nproj", type=int, default=720)
-    ap.add_argument("--grid", type=int, default=1024)
+    ap.add_argument("--det", type=int, default=None, help="square detector size (default 1024)")
+    ap.add_argument("--nproj", type=int, default=None, help="projections (default: the workload's)")
+    ap.add_argument("--grid", type=int, default=None, help="cubic grid size (default: the workload's)")
     ap.add_argument("--no-precision", action="store_true", help="skip the FP64 reference and FP16 error")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--only", choices=["f32", "f16", "both"], default="both")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end run from pinned host memory")
     ap.add_argument("--no-roi", action="store_true", help="skip the region-of-interest run")
+    ap.add_argument("--workload", default="fdk1024",
+                    choices=["fdk1024", "precision2048", "gradient512", "refine768"],
+                    help="fdk1024: the headline line below; the others: bench_workloads.py")
+    ap.add_argument("--det-w", type=int, default=0, help="detector width (other workloads)")
+    ap.add_argument("--det-h", type=int, default=0, help="detector height (other workloads)")
+    ap.add_argument("--slab", type=int, default=0, help="z-slab height (other workloads)")
     ap.add_argument("--calib-copy-gib", type=float, default=0.0,
                     help="run a device-to-device copy of this size first (HBM counter calibration)")
     args = ap.parse_args()
@@ -107,6 +113,14 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(FDK shards over z-slabs with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload != "fdk1024":
+        import bench_workloads
+        bench_workloads.WORKLOADS[args.workload](args)
+        return
+    args.det = args.det or 1024
+    args.nproj = args.nproj or 720
+    args.grid = args.grid or 1024
 
     import torch
     import ct_amd

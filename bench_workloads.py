@@ -1,0 +1,237 @@
+"""Measurement modes beside bench.py's default FDK line (python bench.py --workload ...).
+
+precision2048  2048^3 FP32 volume from 1440 projections of 2048x1536 pixels (0.2 mm
+               pitch, FCD 5000 px), Hann filter; filtered projections stored in FP32, FP16
+               and BF16 (per-projection power-of-two scales), interpolation and
+               accumulation in FP32; every result against the FP64 GPU reconstruction
+               (relative RMSE and max-abs over all voxels).  The volume is processed in
+               z-slabs.  Phantom: 256 seeded ellipsoids (attenuation 0.01..2.0 per phantom
+               unit, scaled) plus 8 needle rods of radius 2 voxels, noise-free intensities.
+gradient512    512^3 volume, 360 projections of 512x512 (FCD 2500 px): forward projector A,
+               its adjoint A^T, the FDK operator and its backward, with and without
+               z-slab recomputation; dot-product test <Ax,y> vs <x,A^T y> on N(0,1) data.
+refine768      sparse view: 120 projections of 1024x768 (FCD 5000 px), 768^3 volume,
+               128 seeded ellipsoids, Poisson counts at I0 = 5e3.  FDK seed from FP16
+               filtered projections, then 10 steepest-descent steps (exact line search) on
+               0.5 ||A x - p||^2 with the adjoint pair, in z-slabs; RMSE against the
+               voxelised analytic phantom and against the plain FP32 FDK, ms per step.
+
+Each mode prints one JSON line.  Synthetic data are generated on the GPU.
+"""
+from __future__ import annotations
+
+import json
+import math
+import sys
+import time
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def _sync(torch):
+    torch.cuda.synchronize()
+
+
+def _errors(torch, v, ref64):
+    """(sum of squared differences, sum of squared reference, max |difference|) in FP64."""
+    d = v.double() - ref64
+    return float(torch.sum(d * d)), float(torch.sum(ref64 * ref64)), float(torch.max(torch.abs(d)))
+
+
+def precision2048(args):
+    import torch
+    import ct_amd
+    from ct_amd.phantom import circular_angles, needles, phantom_unit_radius, random_ellipsoids
+
+    dev = torch.device("cuda:0")
+    W, H, P, n = args.det_w or 2048, args.det_h or 1536, args.nproj or 1440, args.grid or 2048
+    fcd = 500.0 / (0.2 * 500.0 / 1000.0)
+    slab = min(args.slab or 256, n)
+    g = ct_amd.geometry_grid(W, H, fcd, 0.0, n, n, n)
+    angles = circular_angles(P)
+    ctx = ct_amd.Context(0)
+    ctx.set_geometry(g)
+    ctx.set_projections(angles)
+    unit = phantom_unit_radius(g.radius, H, fcd)
+    k = 0.25 / unit
+    ell = [e[:9] + [e[9] * k] for e in random_ellipsoids(256, 3, unit, mu_range=(0.01, 2.0))]
+    ell += needles(8, 33, unit, mu=2.0 * k * 4.0)
+    raw = torch.empty((P, H, W), dtype=torch.int16, device=dev)  # uint16 bytes
+    t0 = time.perf_counter()
+    ctx.phantom_images(ell, 0, P, 0.0, 65535.0, 3, raw, None)
+    _sync(torch)
+    imin = int(torch.min(raw[:, ::8, ::8].to(torch.int32) & 0xFFFF))
+    log(f"phantom {P}x{H}x{W}: {time.perf_counter() - t0:.1f} s, max line integral {-math.log(max(imin, 1) / 65535.0):.2f}")
+    stream = torch.cuda.current_stream().cuda_stream
+    filt, base = "hann", 1.0
+
+    stores = ["f32", "f16", "bf16"]
+    entries, scales, t_pre = {}, {}, {}
+    for s in stores:
+        entries[s] = torch.empty(ctx.filtered_bytes(s, P), dtype=torch.uint8, device=dev)
+        scales[s] = torch.empty(P, dtype=torch.float32, device=dev)
+        _sync(torch)
+        t0 = time.perf_counter()
+        ctx.preprocess(raw, "u16", P, base, "cpu", filt, s, entries[s], scales[s], stream)
+        _sync(torch)
+        t_pre[s] = time.perf_counter() - t0
+        log(f"{s}: preprocessing {t_pre[s] * 1e3:.0f} ms, {entries[s].numel() / 2**30:.1f} GiB of entries")
+
+    ctx64 = ct_amd.Context(0)
+    ctx64.set_geometry(g)
+    ctx64.set_projections(angles)
+    acc = {s: [0.0, 0.0, 0.0] for s in stores}
+    t_bp = {s: 0.0 for s in stores}
+    t64 = 0.0
+    vol64 = torch.empty((n, n, slab), dtype=torch.float64, device=dev)
+    vol = torch.empty((n, n, slab), dtype=torch.float32, device=dev)
+    for z0 in range(0, n, slab):
+        nz = min(slab, n - z0)
+        v64 = vol64[:, :, :nz] if nz == slab else torch.empty((n, n, nz), dtype=torch.float64, device=dev)
+        _sync(torch)
+        t0 = time.perf_counter()
+        ctx64.fdk_f64(raw, "u16", P, base, filt, v64, z0, nz, stream)
+        _sync(torch)
+        t64 += time.perf_counter() - t0
+        for s in stores:
+            v = vol[:, :, :nz] if nz == slab else torch.empty((n, n, nz), dtype=torch.float32, device=dev)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            ctx.backproject(entries[s], s, scales[s], 0, P, v, z0, nz, False, stream)
+            e1.record()
+            _sync(torch)
+            t_bp[s] += e0.elapsed_time(e1) / 1e3
+            d2, r2, mx = _errors(torch, v.contiguous() if nz != slab else v, v64.contiguous() if nz != slab else v64)
+            acc[s][0] += d2
+            acc[s][1] += r2
+            acc[s][2] = max(acc[s][2], mx)
+        log(f"slab z {z0}..{z0 + nz}: FP64 {t64:.1f} s cumulative")
+    updates = float(n) * n * n * P
+    out = {"workload": f"precision: {n}^3 from {P}x{H}x{W}, hann, z-slabs of {slab}", "updates": updates,
+           "fp64_seconds": t64, "stores": {}}
+    for s in stores:
+        out["stores"][s] = {"rel_rmse_vs_f64": math.sqrt(acc[s][0] / acc[s][1]), "maxabs_vs_f64": acc[s][2],
+                            "bp_seconds": t_bp[s], "bp_gups": updates / t_bp[s] / 1e9,
+                            "preprocess_seconds": t_pre[s]}
+    print(json.dumps(out), flush=True)
+
+
+def gradient512(args):
+    import torch
+    import ct_amd
+    from ct_amd.autograd import ConeBeamOperator
+    from ct_amd.phantom import circular_angles
+
+    dev = torch.device("cuda:0")
+    n, W, P = args.grid or 512, args.det_w or 512, args.nproj or 360
+    g = ct_amd.geometry_grid(W, W, 2500.0, 0.0, n, n, n)
+    angles = circular_angles(P)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(4)
+    x = torch.randn((n, n, n), generator=gen, device=dev)
+    gen.manual_seed(5)
+    y = torch.randn((P, W, W), generator=gen, device=dev)
+    res = {"workload": f"differentiation: {n}^3, {P}x{W}x{W}", "updates": float(n) ** 3 * P}
+    for label, kw in (("whole", {}), ("slabs", {"slab_nz": 128, "max_entry_bytes": 256 << 20})):
+        op = ConeBeamOperator(g, angles, filt="ramlak", **kw)
+        times = {}
+        for name, fn, arg in (("A", op.project, x), ("At", op.backproject, y), ("fdk", op.fdk, y),
+                              ("fdk_adjoint", op.fdk_adjoint, x)):
+            fn(arg)
+            _sync(torch)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                r = fn(arg)
+            _sync(torch)
+            times[name] = (time.perf_counter() - t0) / args.steps * 1e3
+            del r
+        ax, aty = op.project(x), op.backproject(y)
+        lhs = float(torch.sum(ax.double() * y.double()))
+        rhs = float(torch.sum(x.double() * aty.double()))
+        res[label] = {"ms": times, "A_gups": res["updates"] / times["A"] / 1e6,
+                      "At_gups": res["updates"] / times["At"] / 1e6,
+                      "dot_rel": abs(lhs - rhs) / max(abs(lhs), abs(rhs))}
+        log(f"{label}: {times}, dot test {res[label]['dot_rel']:.2e}")
+        del op, ax, aty
+    print(json.dumps(res), flush=True)
+
+
+def refine768(args):
+    import torch
+    import ct_amd
+    from ct_amd.autograd import ConeBeamOperator
+    from ct_amd.phantom import circular_angles, phantom_unit_radius, random_ellipsoids, voxelize
+
+    dev = torch.device("cuda:0")
+    W, H, P, n = args.det_w or 1024, args.det_h or 768, args.nproj or 120, args.grid or 768
+    fcd = 500.0 / (0.2 * 500.0 / 1000.0)
+    g = ct_amd.geometry_grid(W, H, fcd, 0.0, n, n, n)
+    angles = circular_angles(P)
+    ctx = ct_amd.Context(0)
+    ctx.set_geometry(g)
+    ctx.set_projections(angles)
+    unit = phantom_unit_radius(g.radius, H, fcd)
+    # attenuation 0.1..1.0 per phantom unit, halved: line integrals of a few units, so
+    # that I0 = 5e3 photons leave tens to hundreds of counts behind the densest paths
+    ell = [e[:9] + [e[9] * 0.5 / unit] for e in random_ellipsoids(128, 7, unit)]
+    photons = 5.0e3
+    raw = torch.empty((P, H, W), dtype=torch.int16, device=dev)
+    ctx.phantom_images(ell, 0, P, photons, 65535.0, 7, raw, None)
+    base = photons / 65535.0
+    stream = torch.cuda.current_stream().cuda_stream
+    truth = voxelize(ell, g, dev)
+    r2 = g.radius * g.radius
+    xs = torch.arange(n, device=dev, dtype=torch.float32) - g.x_pre
+    ys = torch.arange(n, device=dev, dtype=torch.float32) - g.y_pre
+    mask = ((xs[:, None] ** 2 + ys[None, :] ** 2) < r2)[:, :, None].expand(n, n, n)
+
+    def rmse(a, b):
+        d = (a - b)[mask].double()
+        return float(torch.sqrt(torch.mean(d * d)))
+
+    # FDK in the reference's units, then the scale to attenuation per voxel
+    scale = (math.pi / P) * (W / 2 + 1) / (W * W)
+    fdk = {}
+    for s in ("f32", "f16"):
+        ent = torch.empty(ctx.filtered_bytes(s, P), dtype=torch.uint8, device=dev)
+        sc = torch.empty(P, dtype=torch.float32, device=dev)
+        v = torch.empty((n, n, n), dtype=torch.float32, device=dev)
+        ctx.preprocess(raw, "u16", P, base, "cpu", "ramlak", s, ent, sc, stream)
+        ctx.backproject(ent, s, sc, 0, P, v, 0, n, False, stream)
+        fdk[s] = v * scale
+        del ent, sc, v
+    # measured line integrals, the data term of the refinement
+    lineint = torch.empty((P, H, W), dtype=torch.float32, device=dev)
+    raw_f = (raw.to(torch.int32) & 0xFFFF).to(torch.float32)
+    lineint.copy_(-torch.log(torch.clamp(raw_f, min=1.0) / photons))
+    op = ConeBeamOperator(g, angles, slab_nz=args.slab or 256, max_entry_bytes=512 << 20)
+    x = fdk["f16"].clone()
+    hist = [{"step": 0, "rmse_truth": rmse(x, truth), "rmse_vs_fp32_fdk": rmse(x, fdk["f32"])}]
+    step_ms = []
+    for it in range(1, 11):
+        _sync(torch)
+        t0 = time.perf_counter()
+        r = op.project(x) - lineint
+        grad = op.backproject(r)
+        ag = op.project(grad)
+        alpha = float(torch.sum(grad.double() * grad.double()) / torch.sum(ag.double() * ag.double()))
+        x -= alpha * grad
+        _sync(torch)
+        step_ms.append((time.perf_counter() - t0) * 1e3)
+        loss = 0.5 * float(torch.sum(r.double() * r.double()))
+        hist.append({"step": it, "loss_before": loss, "rmse_truth": rmse(x, truth),
+                     "rmse_vs_fp32_fdk": rmse(x, fdk["f32"])})
+        log(f"step {it}: loss {loss:.4e}, RMSE vs phantom {hist[-1]['rmse_truth']:.4e}, {step_ms[-1]:.0f} ms")
+    out = {"workload": f"sparse-view refinement: {n}^3 from {P}x{H}x{W}, Poisson I0=5e3, FP16 FDK seed, "
+                       f"10 steepest-descent steps in z-slabs of {op.slab_nz}",
+           "fdk_scale_to_mu_per_voxel": scale,
+           "rmse_fp32_fdk_vs_truth": rmse(fdk["f32"], truth), "rmse_fp16_fdk_vs_truth": rmse(fdk["f16"], truth),
+           "truth_rms": float(torch.sqrt(torch.mean(truth[mask].double() ** 2))),
+           "ms_per_step": sum(step_ms) / len(step_ms), "steps": hist}
+    print(json.dumps(out), flush=True)
+
+
+WORKLOADS = {"precision2048": precision2048, "gradient512": gradient512, "refine768": refine768}

@@ -5,9 +5,8 @@
 // voxel-driven backprojection of Ct/CtVolume.cpp:979-1039 term by term: every voxel that
 // the backprojection would read through taps (u0,v0),(u0+1,v0),(u0,v0+1),(u0+1,v0+1) with
 // weight U^2 and bilinear weights instead adds U^2 * f * weight into those pixels, under
-// the same cylinder and window masks.  A workgroup accumulates its block's splats for
-// one projection in an LDS tile (ds_add_f32), then adds the tile to the detector image
-// with global float atomics.
+// the same cylinder and window masks.  It is computed as a gather (fp_gather_kernel):
+// each detector pixel sums the taps that land on it.
 #include "ctg_internal.h"
 
 #include <algorithm>
@@ -16,143 +15,113 @@
 namespace ctg {
 namespace {
 
-constexpr int kTX = 16, kTY = 16, kThreads = 256, kZR = 16;
-static_assert(kRowAnchor % kZR == 0, "a forward-projection run lies inside one row anchor block (det_row_index)");
-constexpr int kFpCap = 4096;  // LDS accumulation tile capacity (floats)
-
 struct FpArgs {
-    int W, H, nx, ny, nzs, z0, nproj, ntx, nty;
-    int zshift;  // z0 mod 16: runs start at grid z multiples of 16
+    int W, H, nx, ny, nzs, z0, nproj;
     float xpre, ypre, zpre, upre, vpre;
     float r2, fcd, uoff, ulo, uhi, vlo, vhi;
 };
 
-__global__ __launch_bounds__(kThreads) void fp_kernel(FpArgs a, const float4* __restrict__ pp,
-                                                      const float* __restrict__ vol,
-                                                      float* __restrict__ proj) {
-    __shared__ float acc[kFpCap];
-    const int tile_id = blockIdx.x;
-    const int ty = tile_id / a.ntx, tx = tile_id - ty * a.ntx;
-    const int zb = blockIdx.y * kZR;
-    const int tid = threadIdx.x;
-    const int xi = tx * kTX + (tid & (kTX - 1));
-    const int yi = ty * kTY + (tid / kTX);
-    // run-local k is a slab voxel when zfirst <= k < zlast (slab index zb - zshift + k)
-    const int zfirst = a.zshift - zb;
-    const int zlast = a.nzs + a.zshift - zb;
-    const bool colvol = xi < a.nx && yi < a.ny;
-    const float x = (float)xi - a.xpre, y = (float)yi - a.ypre;
-    const bool incyl = colvol && (x * x + y * y < a.r2);
+// Gather form of the projector.  No atomics at all: scattering splats costs LDS float
+// atomics (about 30x an LDS read here) or global ones that contend across workgroups.
+// A workgroup owns one detector column c of one projection and kGRows rows, two
+// adjacent pixels (r, r + 1) per lane, accumulated in registers.
+//  - All lanes walk the same voxel columns: exactly those whose ray lands within
+//    [c - 1, c + 1) on the detector (u0 = c takes weight 1 - fu, u0 = c - 1 takes fu).
+//    Along a line of constant outer coordinate u is monotonic in the inner one, so the
+//    two rays from the source through u = c - 1 and u = c + 1 (plus 0.05 pixel) bound
+//    the candidates up to rounding; a 0.01-voxel margin covers that.
+//  - For a voxel column, a lane tests the z whose detector row index lies in
+//    [r - 1, r + 2) (about 3 / U of them) with the backprojection's own arithmetic;
+//    v0 = r - 1, r, r + 1 feed rows r and r + 1 with the bilinear row weights.
+// Every (voxel, tap) pair is therefore added exactly once, by the lane that owns the
+// tap's pixel, and the result does not depend on scheduling.
+constexpr int kGThreads = 256, kGRows = 2 * kGThreads;
 
-    float f[kZR];
-    {
-        const float* col = vol + ((size_t)(colvol ? xi : 0) * a.ny + (colvol ? yi : 0)) * a.nzs;
-#pragma unroll
-        for (int k = 0; k < kZR; ++k) f[k] = (incyl && k >= zfirst && k < zlast) ? col[zb - a.zshift + k] : 0.0f;
-    }
-    const float cxa = (float)(tx * kTX) - a.xpre;
-    const float cxb = (float)min(tx * kTX + kTX - 1, a.nx - 1) - a.xpre;
-    const float cya = (float)(ty * kTY) - a.ypre;
-    const float cyb = (float)min(ty * kTY + kTY - 1, a.ny - 1) - a.ypre;
-    const float ndx = cxa > 0.0f ? cxa : (cxb < 0.0f ? cxb : 0.0f);
-    const float ndy = cya > 0.0f ? cya : (cyb < 0.0f ? cyb : 0.0f);
-    if (!(ndx * ndx + ndy * ndy < a.r2)) return;  // uniform: no voxel of the tile in the cylinder
-    // detector rows come from the run's 32-slice anchor (det_row_index): k + koff past it
-    const int zrun = a.z0 - a.zshift + zb;
-    const float zw0 = (float)zrun - a.zpre;
-    const float zw1 = zw0 + (float)(kZR - 1);
-    const int koff = zrun & (kRowAnchor - 1);
-    const float zwa = (float)(zrun - koff) - a.zpre;
+__global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const float4* __restrict__ pp,
+                                                              const float* __restrict__ vol,
+                                                              float* __restrict__ proj) {
+    const int c = blockIdx.x;
+    const int r0 = blockIdx.y * kGRows + 2 * (int)threadIdx.x;
+    const int p = blockIdx.z;
+    const float4 g = pp[p];
+    const float sn = g.x, cs = g.y, h = g.z;
+    // source and detector points in world xy (voxel units; the virtual detector passes
+    // through the axis, lateral coordinate u)
+    const float Sx = a.fcd * cs + a.uoff * sn, Sy = a.fcd * sn - a.uoff * cs;
+    const float ulo = (float)c - 1.05f - a.upre, uhi = (float)c + 1.05f - a.upre;
+    const float q0x = -(ulo - a.uoff) * sn, q0y = (ulo - a.uoff) * cs;
+    const float q1x = -(uhi - a.uoff) * sn, q1y = (uhi - a.uoff) * cs;
+    const float umid = (float)c - a.upre - a.uoff;
+    const bool xmajor = fabsf(-umid * sn - Sx) >= fabsf(umid * cs - Sy);
+    const int n_out = xmajor ? a.nx : a.ny, n_in = xmajor ? a.ny : a.nx;
+    const float pre_out = xmajor ? a.xpre : a.ypre, pre_in = xmajor ? a.ypre : a.xpre;
+    const float So = xmajor ? Sx : Sy, Si = xmajor ? Sy : Sx;
+    const float k0 = ((xmajor ? q0y : q0x) - Si) / ((xmajor ? q0x : q0y) - So);
+    const float k1 = ((xmajor ? q1y : q1x) - Si) / ((xmajor ? q1x : q1y) - So);
+    const float R = sqrtf(a.r2);
+    const int o_lo = max(0, (int)floorf(pre_out - R) - 1), o_hi = min(n_out - 1, (int)ceilf(pre_out + R) + 1);
+    const int zend = a.z0 + a.nzs - 1;  // last grid z index of the slab
+    const float rf = (float)r0;
 
-    for (int p = 0; p < a.nproj; ++p) {
-        const float4 g = pp[p];
-        // footprint box of the block in pixels (same construction as the backprojection)
-        float umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
-        bool bad = false;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const float cx = (c & 1) ? cxb : cxa;
-            const float cy = (c & 2) ? cyb : cya;
-            float t = (-1.0f) * cx * g.x + cy * g.y;
+    float acc0 = 0.0f, acc1 = 0.0f;
+    for (int o = o_lo; o <= o_hi; ++o) {
+        const float ow = (float)o - pre_out;
+        const float i0 = Si + (ow - So) * k0, i1 = Si + (ow - So) * k1;
+        const int j_lo = max(0, (int)ceilf(fminf(i0, i1) + pre_in - 0.01f));
+        const int j_hi = min(n_in - 1, (int)floorf(fmaxf(i0, i1) + pre_in + 0.01f));
+        for (int j = j_lo; j <= j_hi; ++j) {
+            const int xi = xmajor ? o : j, yi = xmajor ? j : o;
+            // per-voxel arithmetic identical to the backprojection's
+            const float x = (float)xi - a.xpre, y = (float)yi - a.ypre;
+            if (!(x * x + y * y < a.r2)) continue;
+            float t = (-1.0f) * x * g.x + y * g.y;
             t += a.uoff;
-            const float s = cx * g.y + cy * g.x;
-            const float den = a.fcd - s;
-            bad |= !(den > 0.0f);
-            const float U = a.fcd / den;
-            const float ui = t * U + a.upre;
-            const float va = (-1.0f) * ((zw0 + g.z) * U) + a.vpre;
-            const float vb = (-1.0f) * ((zw1 + g.z) * U) + a.vpre;
-            umin = fminf(umin, ui); umax = fmaxf(umax, ui);
-            vmin = fminf(vmin, fminf(va, vb)); vmax = fmaxf(vmax, fmaxf(va, vb));
-        }
-        int us = 0, vs = 0, ncol = 0, nrow = 0;
-        bool use_lds = false;
-        if (!bad && umin == umin && vmin == vmin) {
-            us = max((int)floorf(umin) - 1, 0);
-            const int ue = min((int)floorf(umax) + 2, a.W - 1);
-            vs = max((int)floorf(vmin) - 1, 0);
-            const int ve = min((int)floorf(vmax) + 2, a.H - 1);
-            ncol = ue - us + 1;
-            nrow = ve - vs + 1;
-            if (ncol <= 0 || nrow <= 0) continue;  // uniform: nothing of this block hits the detector
-            use_lds = ncol * nrow <= kFpCap;
-        }
-        float* img = proj + (size_t)p * a.W * a.H;
-        if (use_lds) {
-            for (int e = tid; e < ncol * nrow; e += kThreads) acc[e] = 0.0f;
-            __syncthreads();
-        }
-        float t = (-1.0f) * x * g.x + y * g.y;
-        t += a.uoff;
-        const float s = x * g.y + y * g.x;
-        const float U = cone_magnification(a.fcd, s);
-        const float u = t * U;
-        const bool cv = incyl && u >= a.ulo && u <= a.uhi;
-        if (cv) {
+            const float s = x * g.y + y * g.x;
+            const float U = cone_magnification(a.fcd, s);
+            const float u = t * U;
+            if (!(u >= a.ulo && u <= a.uhi)) continue;
             const float uIdx = u + a.upre;
             const int u0 = (int)uIdx;
+            if (u0 != c && u0 != c - 1) continue;
             const float fu = uIdx - (float)u0;
-            const float w = U * U;
-            const float vA = det_row_anchor(zwa, g.z, U, a.vpre);
-#pragma unroll
-            for (int k = 0; k < kZR; ++k) {
-                const float zk = zw0 + (float)k;
-                const float v = (zk + g.z) * U;
-                if (k >= zfirst && k < zlast && v >= a.vlo && v <= a.vhi) {
-                    const float vI = det_row_index((float)(k + koff), U, vA);
-                    const int v0 = (int)vI;
-                    const float fv = vI - (float)v0;
-                    const float wf = w * f[k];
-                    const float b0 = wf * (1.0f - fv), b1 = wf * fv;
-                    const float c00 = b0 * (1.0f - fu), c10 = b0 * fu, c01 = b1 * (1.0f - fu), c11 = b1 * fu;
-                    if (use_lds) {
-                        const int e = (u0 - us) * nrow + (v0 - vs);
-                        atomicAdd(&acc[e], c00);
-                        atomicAdd(&acc[e + nrow], c10);
-                        atomicAdd(&acc[e + 1], c01);
-                        atomicAdd(&acc[e + nrow + 1], c11);
-                    } else {
-                        float* r0 = img + (size_t)v0 * a.W + u0;
-                        atomicAdd(r0, c00);
-                        atomicAdd(r0 + 1, c10);
-                        atomicAdd(r0 + a.W, c01);
-                        atomicAdd(r0 + a.W + 1, c11);
-                    }
+            const float wc = U * U * (u0 == c ? 1.0f - fu : fu);
+            // z whose row index lies in [r0 - 1, r0 + 2): vIndex ~ vPre - (z - zPre + h) U
+            const float iU = __builtin_amdgcn_rcpf(U);
+            const float za_f = (a.vpre - rf - 2.0f) * iU - h + a.zpre;
+            const float zb_f = (a.vpre - rf + 1.0f) * iU - h + a.zpre;
+            const int zlo = max(a.z0, (int)ceilf(za_f - 0.01f));
+            const int zhi = min(zend, (int)floorf(zb_f + 0.01f));
+            const float* colv = vol + ((size_t)xi * a.ny + yi) * a.nzs - a.z0;
+            int za_cur = -1;
+            float vA = 0.0f;
+            for (int zi = zlo; zi <= zhi; ++zi) {
+                const float zk = (float)zi - a.zpre;
+                const float v = (zk + h) * U;
+                if (!(v >= a.vlo && v <= a.vhi)) continue;
+                const int za = zi & ~(kRowAnchor - 1);
+                if (za != za_cur) {
+                    za_cur = za;
+                    vA = det_row_anchor((float)za - a.zpre, h, U, a.vpre);
+                }
+                const float vI = det_row_index((float)(zi - za), U, vA);
+                const int v0 = (int)vI;
+                const float fv = vI - (float)v0;
+                const float val = wc * colv[zi];
+                const int d = v0 - r0;
+                if (d == 0) {
+                    acc0 = fmaf(1.0f - fv, val, acc0);
+                    acc1 = fmaf(fv, val, acc1);
+                } else if (d == -1) {
+                    acc0 = fmaf(fv, val, acc0);
+                } else if (d == 1) {
+                    acc1 = fmaf(1.0f - fv, val, acc1);
                 }
             }
-        }
-        if (use_lds) {
-            __syncthreads();
-            for (int e = tid; e < ncol * nrow; e += kThreads) {
-                const float val = acc[e];
-                if (val != 0.0f) {
-                    const int c = e / nrow, r = e - (e / nrow) * nrow;
-                    atomicAdd(img + (size_t)(vs + r) * a.W + (us + c), val);
-                }
-            }
-            __syncthreads();
         }
     }
+    float* px = proj + ((size_t)p * a.H + r0) * a.W + c;
+    if (r0 < a.H) px[0] += acc0;
+    if (r0 + 1 < a.H) px[a.W] += acc1;
 }
 
 // ------------------------------------------------------------------ phantom
@@ -287,13 +256,12 @@ ctg_status launch_forward_project(const ctg_geometry& g, const float4* d_params,
     if (nz <= 0 || nproj <= 0) return CTG_OK;
     FpArgs a{};
     a.W = g.det_width; a.H = g.det_height; a.nx = g.nx; a.ny = g.ny; a.nzs = nz; a.z0 = z0; a.nproj = nproj;
-    a.zshift = z0 & 15;
-    a.ntx = (g.nx + kTX - 1) / kTX; a.nty = (g.ny + kTY - 1) / kTY;
     a.xpre = g.x_pre; a.ypre = g.y_pre; a.zpre = g.z_pre; a.upre = g.u_pre; a.vpre = g.v_pre;
     a.r2 = g.radius * g.radius; a.fcd = g.fcd; a.uoff = g.u_offset;
     a.ulo = g.u_lo; a.uhi = g.u_hi; a.vlo = g.v_lo; a.vhi = g.v_hi;
-    dim3 grid((unsigned)(a.ntx * a.nty), (unsigned)((nz + a.zshift + kZR - 1) / kZR));
-    hipLaunchKernelGGL(fp_kernel, grid, dim3(kThreads), 0, st, a, d_params, d_volume, d_proj);
+    dim3 grid((unsigned)a.W, (unsigned)((a.H + kGRows - 1) / kGRows), (unsigned)nproj);
+    if (grid.y > 65535u || grid.z > 65535u) return fail(CTG_ERROR_UNSUPPORTED, "forward projection grid too large");
+    hipLaunchKernelGGL(fp_gather_kernel, grid, dim3(kGThreads), 0, st, a, d_params, d_volume, d_proj);
     CTG_HIP(hipGetLastError());
     return CTG_OK;
 }
