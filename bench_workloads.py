@@ -209,6 +209,11 @@ def refine768(args):
     lineint.copy_(-torch.log(torch.clamp(raw_f, min=1.0) / photons))
     op = ConeBeamOperator(g, angles, slab_nz=args.slab or 256, max_entry_bytes=512 << 20)
     x = fdk["f16"].clone()
+    # least-squares scale of the seed against the data: how well the FDK scale constant
+    # matches the projector's normalisation (1 = exactly)
+    ax = op.project(x)
+    ls_scale = float(torch.sum(ax.double() * lineint.double()) / torch.sum(ax.double() * ax.double()))
+    del ax
     hist = [{"step": 0, "rmse_truth": rmse(x, truth), "rmse_vs_fp32_fdk": rmse(x, fdk["f32"])}]
     step_ms = []
     for it in range(1, 11):
@@ -227,7 +232,8 @@ def refine768(args):
         log(f"step {it}: loss {loss:.4e}, RMSE vs phantom {hist[-1]['rmse_truth']:.4e}, {step_ms[-1]:.0f} ms")
     out = {"workload": f"sparse-view refinement: {n}^3 from {P}x{H}x{W}, Poisson I0=5e3, FP16 FDK seed, "
                        f"10 steepest-descent steps in z-slabs of {op.slab_nz}",
-           "fdk_scale_to_mu_per_voxel": scale,
+           "fdk_scale_to_mu_per_voxel": scale, "fdk_seed_least_squares_scale": ls_scale,
+           "rmse_ls_scaled_fdk_vs_truth": rmse(fdk["f16"] * ls_scale, truth),
            "rmse_fp32_fdk_vs_truth": rmse(fdk["f32"], truth), "rmse_fp16_fdk_vs_truth": rmse(fdk["f16"], truth),
            "truth_rms": float(torch.sqrt(torch.mean(truth[mask].double() ** 2))),
            "ms_per_step": sum(step_ms) / len(step_ms), "steps": hist}
