@@ -83,6 +83,8 @@ SIGNATURES = {
     "ctg_cancel": (None, [c_void_p]),
     "ctg_reset_cancel": (None, [c_void_p]),
     "ctg_set_spare_memory": (c_int32, [c_void_p, c_uint64]),
+    "ctg_set_memory_budget": (c_int32, [c_void_p, c_uint64]),
+    "ctg_plan_reconstruct": (c_int32, [c_void_p, c_int32, c_int32, c_int32, POINTER(c_int32), POINTER(c_int32)]),
     "ctg_filtered_bytes": (c_size_t, [POINTER(Geometry), c_int32, c_int32]),
     "ctg_preprocess": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_float, c_int32, c_int32, c_int32,
                                  c_void_p, c_void_p, c_void_p]),
@@ -221,6 +223,16 @@ class Context:
 
     def set_spare_memory(self, mib: int):
         check(self.lib.ctg_set_spare_memory(self.handle, int(mib)))
+
+    def set_memory_budget(self, nbytes: int):
+        check(self.lib.ctg_set_memory_budget(self.handle, int(nbytes)))
+
+    def plan_reconstruct(self, pixel, nproj, store):
+        """(resident, slab_nz) that reconstruct() would use under the current memory limits."""
+        r, z = c_int32(), c_int32()
+        check(self.lib.ctg_plan_reconstruct(self.handle, PIXEL[pixel], nproj, STORE[store], ctypes.byref(r),
+                                            ctypes.byref(z)))
+        return bool(r.value), z.value
 
     def filtered_bytes(self, store: str, nproj: int) -> int:
         return self.lib.ctg_filtered_bytes(ctypes.byref(self.geometry), STORE[store], nproj)

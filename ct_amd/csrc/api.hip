@@ -345,6 +345,12 @@ ctg_status ctg_set_spare_memory(ctg_context* ctx, uint64_t mib) {
     return CTG_OK;
 }
 
+ctg_status ctg_set_memory_budget(ctg_context* ctx, uint64_t bytes) {
+    if (!ctx) return fail(CTG_ERROR_INVALID_ARGUMENT, "null context");
+    ctx->budget_bytes = bytes;
+    return CTG_OK;
+}
+
 size_t ctg_filtered_bytes(const ctg_geometry* g, ctg_store store, int32_t nproj) {
     if (!g || nproj < 0) return 0;
     return (size_t)g->det_width * g->det_height * store_bytes(store) * (size_t)nproj;
@@ -499,6 +505,65 @@ ctg_status ctg_phantom_images(ctg_context* ctx, const double* ell, int32_t n_ell
 // backprojection layout; then every z-slab is backprojected from all projections,
 // converted to the requested index order and copied out.  When the filtered projections
 // do not fit next to a useful slab, they are recomputed per slab from re-uploaded images.
+namespace {
+
+struct ReconPlan {
+    int batch = 1;       // projections uploaded and preprocessed per step
+    bool resident = true;
+    int slab_nz = 0;
+};
+
+// Memory plan of ctg_reconstruct: the working set (staging and FP32 rows of one batch),
+// the filtered projections (all of them when they fit next to a useful slab, else one
+// batch, recomputed per slab), and the largest slab (z-slices of the volume plus its
+// index-order conversion) that fits the rest.  Available memory is the device's free
+// memory minus the spare memory, capped by the budget; buffers the context already
+// holds count as available because they are reused.
+ctg_status plan_reconstruct(ctg_context* ctx, ctg_pixel_type pixel, int nproj, ctg_store store, ReconPlan& plan) {
+    const ctg_geometry& g = ctx->geom;
+    const size_t img = (size_t)g.det_width * g.det_height;
+    const size_t pb = pixel_bytes(pixel);
+    const size_t col = (size_t)g.nx * g.ny;
+    size_t fr = 0, tot = 0;
+    CTG_HIP(hipMemGetInfo(&fr, &tot));
+    const long long spare = (long long)ctx->spare_mib * 1024ll * 1024ll;
+    long long avail = (long long)fr - spare;
+    avail += (long long)ctx->filtered.bytes + (long long)ctx->raw_stage.bytes + (long long)ctx->rows_f32.bytes;
+    if (ctx->budget_bytes > 0) avail = std::min<long long>(avail, (long long)ctx->budget_bytes);
+    plan.batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nproj, ((size_t)256 << 20) / (img * 4)));
+    const long long work = (long long)img * plan.batch * (4 + 2 * pb) + (long long)nproj * 8;
+    const long long filt = (long long)ctg_filtered_bytes(&g, store, nproj);
+    const long long slab_col = (long long)col * 4 * 2;  // slab + converted copy, per z
+    plan.resident = avail - work - filt >= slab_col * std::min(g.nz, 32);
+    const long long for_slabs =
+        avail - work - (plan.resident ? filt : (long long)ctg_filtered_bytes(&g, store, plan.batch));
+    if (for_slabs < slab_col) {
+        return fail(CTG_ERROR_INSUFFICIENT_VRAM,
+                    "The VRAM of the GPU is insufficient to run a reconstruction (free %zu MiB, spare %llu MiB).",
+                    fr >> 20, (unsigned long long)ctx->spare_mib);
+    }
+    plan.slab_nz = (int)std::min<long long>(g.nz, for_slabs / slab_col);
+    // split volumes use slabs of whole 32-slice runs (the backprojection's run length)
+    if (plan.slab_nz < g.nz && plan.slab_nz >= 32) plan.slab_nz -= plan.slab_nz % 32;
+    return CTG_OK;
+}
+
+}  // namespace
+
+ctg_status ctg_plan_reconstruct(ctg_context* ctx, ctg_pixel_type pixel, int32_t nproj, ctg_store store,
+                                int32_t* resident, int32_t* slab_nz) {
+    clear_error();
+    CTG_TRY(check_ctx(ctx));
+    CTG_TRY(check_range(ctx, 0, nproj));
+    if ((int)pixel < 0 || (int)pixel > 2 || (int)store < 0 || (int)store > 2 || nproj <= 0)
+        return fail(CTG_ERROR_INVALID_ARGUMENT, "bad pixel type, store or projection count");
+    ReconPlan plan;
+    CTG_TRY(plan_reconstruct(ctx, pixel, nproj, store, plan));
+    if (resident) *resident = plan.resident ? 1 : 0;
+    if (slab_nz) *slab_nz = plan.slab_nz;
+    return CTG_OK;
+}
+
 ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type pixel, int32_t nproj,
                            float base_intensity, ctg_log_form log_form, ctg_filter filter, ctg_store store,
                            float* h_volume, ctg_index_order order, ctg_progress_fn progress, void* user) {
@@ -513,25 +578,12 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
     const size_t col = (size_t)g.nx * g.ny;
     if (col == 0 || g.nz == 0) return CTG_OK;
 
-    size_t fr = 0, tot = 0;
-    CTG_HIP(hipMemGetInfo(&fr, &tot));
-    const long long spare = (long long)ctx->spare_mib * 1024ll * 1024ll;
-    long long avail = (long long)fr - spare;
-    // buffers owned by the context count as available (they are reused)
-    avail += (long long)ctx->filtered.bytes + (long long)ctx->raw_stage.bytes + (long long)ctx->rows_f32.bytes;
-    const int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nproj, ((size_t)256 << 20) / (img * 4)));
-    const long long work = (long long)img * batch * (4 + 2 * pb) + (long long)nproj * 8;
+    ReconPlan plan;
+    CTG_TRY(plan_reconstruct(ctx, pixel, nproj, store, plan));
+    const int batch = plan.batch;
+    const bool resident = plan.resident;
+    const int slab_nz = plan.slab_nz;
     const long long filt = (long long)ctg_filtered_bytes(&g, store, nproj);
-    const long long slab_col = (long long)col * 4 * 2;  // slab + converted copy, per z
-    bool resident = avail - work - filt >= slab_col * std::min(g.nz, 32);
-    long long for_slabs = avail - work - (resident ? filt : (long long)ctg_filtered_bytes(&g, store, batch));
-    if (for_slabs < slab_col) {
-        return fail(CTG_ERROR_INSUFFICIENT_VRAM,
-                    "The VRAM of the GPU is insufficient to run a reconstruction (free %zu MiB, spare %llu MiB).",
-                    fr >> 20, (unsigned long long)ctx->spare_mib);
-    }
-    int slab_nz = (int)std::min<long long>(g.nz, for_slabs / slab_col);
-    if (slab_nz >= 32) slab_nz -= slab_nz % 32;
 
     hipStream_t s_up = nullptr, s_cmp = nullptr;
     hipEvent_t ev_up[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr};
@@ -579,34 +631,52 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
         PIPE_HIP(hipHostMalloc(&pinned[i], img * pb * batch, hipHostMallocDefault));
     }
     PIPE_TRY(ctx->raw_stage.reserve(img * pb * batch * 2));
-    PIPE_TRY(ctx->filtered.reserve((size_t)(resident ? filt : (long long)ctg_filtered_bytes(&g, store, nproj))));
+    // resident: every projection's entries stay in HBM for all slabs; otherwise one
+    // batch of entries is re-derived from its uploaded images for every slab
+    PIPE_TRY(ctx->filtered.reserve((size_t)(resident ? filt : (long long)ctg_filtered_bytes(&g, store, batch))));
     PIPE_TRY(ctx->scales_tmp.reserve(sizeof(float) * nproj));
     PIPE_HIP(hipMalloc((void**)&d_slab, col * 4 * (size_t)slab_nz));
     if (order == CTG_X_FASTEST) PIPE_HIP(hipMalloc(&d_conv, col * 4 * (size_t)slab_nz));
+    const int nb_total = (nproj + batch - 1) / batch;
 
-    // Upload + preprocess all projections (double-buffered), into ctx->filtered.
-    auto preprocess_all = [&]() -> ctg_status {
-        const int nb_total = (nproj + batch - 1) / batch;
-        for (int b = 0; b < nb_total; ++b) {
-            if (ctx->cancel.load()) return fail(CTG_ERROR_INTERRUPTED, "interrupted");
-            const int i = b & 1;
-            const int b0 = b * batch, nb = std::min(batch, nproj - b0);
-            // staging buffer i is free once the preprocessing that read it has finished
-            CTG_HIP(hipEventSynchronize(ev_done[i]));
-            memcpy(pinned[i], (const char*)h_raw + (size_t)b0 * img * pb, (size_t)nb * img * pb);
-            char* dst = (char*)ctx->raw_stage.ptr + (size_t)i * img * pb * batch;
-            CTG_HIP(hipMemcpyAsync(dst, pinned[i], (size_t)nb * img * pb, hipMemcpyHostToDevice, s_up));
-            CTG_HIP(hipEventRecord(ev_up[i], s_up));
-            CTG_HIP(hipStreamWaitEvent(s_cmp, ev_up[i], 0));
-            char* out = (char*)ctx->filtered.ptr + (size_t)b0 * img * store_bytes(store);
-            CTG_TRY(preprocess_batched(ctx, dst, pixel, nb, base_intensity, log_form, filter, store, out,
-                                       ctx->scales_tmp.as<float>() + b0, s_cmp));
-            CTG_HIP(hipEventRecord(ev_done[i], s_cmp));
-        }
+    // Upload batch b through pinned staging (double-buffered on the upload stream) and
+    // preprocess it on the compute stream into `out` (entries of projections b0..).
+    auto stage_batch = [&](int b, char* out) -> ctg_status {
+        if (ctx->cancel.load()) return fail(CTG_ERROR_INTERRUPTED, "interrupted");
+        const int i = b & 1;
+        const int b0 = b * batch, nb = std::min(batch, nproj - b0);
+        // staging buffer i is free once the preprocessing that read it has finished
+        CTG_HIP(hipEventSynchronize(ev_done[i]));
+        memcpy(pinned[i], (const char*)h_raw + (size_t)b0 * img * pb, (size_t)nb * img * pb);
+        char* dst = (char*)ctx->raw_stage.ptr + (size_t)i * img * pb * batch;
+        CTG_HIP(hipMemcpyAsync(dst, pinned[i], (size_t)nb * img * pb, hipMemcpyHostToDevice, s_up));
+        CTG_HIP(hipEventRecord(ev_up[i], s_up));
+        CTG_HIP(hipStreamWaitEvent(s_cmp, ev_up[i], 0));
+        CTG_TRY(preprocess_batched(ctx, dst, pixel, nb, base_intensity, log_form, filter, store, out,
+                                   ctx->scales_tmp.as<float>() + b0, s_cmp));
+        CTG_HIP(hipEventRecord(ev_done[i], s_cmp));
         return CTG_OK;
     };
+    auto backproject = [&](int p0, int np, int z0, int nz, int accumulate) -> ctg_status {
+        BpLaunch L{};
+        L.g = &g;
+        L.d_params = ctx->proj_dev.as<float4>() + p0;
+        L.h_params = ctx->proj_host.data() + p0;
+        L.d_scales = ctx->scales_tmp.as<float>() + p0;
+        L.d_filtered = ctx->filtered.ptr;
+        L.store = store;
+        L.nproj = np;
+        L.d_volume = d_slab;
+        L.z0 = z0;
+        L.nz = nz;
+        L.accumulate = accumulate;
+        return launch_backproject(L, s_cmp);
+    };
 
-    if (resident) PIPE_TRY(preprocess_all());
+    if (resident) {
+        for (int b = 0; b < nb_total; ++b)
+            PIPE_TRY(stage_batch(b, (char*)ctx->filtered.ptr + (size_t)b * batch * img * store_bytes(store)));
+    }
     const int nslabs = (g.nz + slab_nz - 1) / slab_nz;
     for (int s = 0; s < nslabs; ++s) {
         if (ctx->cancel.load()) {
@@ -615,20 +685,17 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
             return st;
         }
         const int z0 = s * slab_nz, nz = std::min(slab_nz, g.nz - z0);
-        if (!resident) PIPE_TRY(preprocess_all());
-        BpLaunch L{};
-        L.g = &g;
-        L.d_params = ctx->proj_dev.as<float4>();
-        L.h_params = ctx->proj_host.data();
-        L.d_scales = ctx->scales_tmp.as<float>();
-        L.d_filtered = ctx->filtered.ptr;
-        L.store = store;
-        L.nproj = nproj;
-        L.d_volume = d_slab;
-        L.z0 = z0;
-        L.nz = nz;
-        L.accumulate = 0;
-        PIPE_TRY(launch_backproject(L, s_cmp));
+        if (resident) {
+            PIPE_TRY(backproject(0, nproj, z0, nz, 0));
+        } else {
+            // the single batch buffer is rewritten only after the previous batch's
+            // backprojection (both on the compute stream)
+            for (int b = 0; b < nb_total; ++b) {
+                const int b0 = b * batch, nb = std::min(batch, nproj - b0);
+                PIPE_TRY(stage_batch(b, (char*)ctx->filtered.ptr));
+                PIPE_TRY(backproject(b0, nb, z0, nz, b > 0 ? 1 : 0));
+            }
+        }
         if (order == CTG_X_FASTEST) {
             PIPE_TRY(launch_convert_volume(g, d_slab, nz, CTG_X_FASTEST, CTG_LITTLE_ENDIAN, d_conv, s_cmp));
             PIPE_HIP(hipMemcpyAsync(h_volume + (size_t)z0 * col, d_conv, col * 4 * (size_t)nz, hipMemcpyDeviceToHost,

@@ -86,6 +86,7 @@ struct ctg_context {
     int nproj_table = 0;
     std::atomic<bool> cancel{false};
     uint64_t spare_mib = 200;        // CtVolume::gpuSpareMemory default (Ct/CtVolume.h:242)
+    uint64_t budget_bytes = 0;       // cap on what a reconstruction plans with (0 = free memory)
     ctg::FilterTables ftab;
     ctg::DevBuf rows_f32;            // row-major FP32 work images
     ctg::DevBuf rows_f64;

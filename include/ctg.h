@@ -131,6 +131,10 @@ void ctg_cancel(ctg_context* ctx);
 void ctg_reset_cancel(ctg_context* ctx);
 /* VRAM to leave free when sizing slabs, in MiB (setGpuSpareMemory, Ct/CtVolume.cpp:522-524). */
 ctg_status ctg_set_spare_memory(ctg_context* ctx, uint64_t mib);
+/* Upper bound, in bytes, on the device memory ctg_reconstruct plans with (0 = the free
+ * memory minus the spare memory).  For sharing a GPU with other work; no counterpart in
+ * the reference, whose only control is the spare memory. */
+ctg_status ctg_set_memory_budget(ctg_context* ctx, uint64_t bytes);
 
 /* ----------------------------------------------------- stream-ordered ops */
 
@@ -219,6 +223,16 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
                            int32_t nproj, float base_intensity, ctg_log_form log_form,
                            ctg_filter filter, ctg_store store, float* h_volume,
                            ctg_index_order order, ctg_progress_fn progress, void* user);
+
+/* The plan ctg_reconstruct would follow for these inputs under the current memory
+ * limits: *resident = 1 when every filtered projection stays in device memory for all
+ * slabs (else each slab re-derives them batch by batch from the uploaded images), and
+ * *slab_nz = z-slices per slab.  Fails with CTG_ERROR_INSUFFICIENT_VRAM, as the
+ * reconstruction would, when no slab fits (getMaxChunkSize() == 0,
+ * Ct/CtVolume.cpp:1120-1126); the reference's counterpart for callers sizing a job is
+ * getRequiredMemoryUpperBound (Ct/CtVolume.cpp:469-480). */
+ctg_status ctg_plan_reconstruct(ctg_context* ctx, ctg_pixel_type pixel, int32_t nproj, ctg_store store,
+                                int32_t* resident, int32_t* slab_nz);
 
 /* ------------------------------------------------------ synthetic inputs */
 

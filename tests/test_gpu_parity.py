@@ -288,3 +288,65 @@ def test_convert_volume_orders_and_byte_swap(gpu):
     ctx.convert_volume(vol, g.nz, "zfastest", "bigendian", out)
     torch.cuda.synchronize()
     assert np.array_equal(out.cpu().numpy().view(">f4").astype(np.float32), v.ravel())
+
+
+def test_reconstruct_memory_plans_slabs_and_recomputation(gpu):
+    """ctg_reconstruct under memory budgets: resident entries with several slabs give the
+    same volume bit for bit; entries re-derived per slab and batch (non-resident) give it
+    to FP32 summation order; no slab fitting and cancellation are reported as errors.
+    1024^2 detector, 80 projections: batches of 64 + 16 projections (256 MiB of FP32 rows)."""
+    import torch
+    import ct_amd
+    from ct_amd import _ctg
+    from ct_amd.phantom import circular_angles, random_ellipsoids
+    W = H = 1024
+    P, n, nz = 80, 64, 48
+    g = ct_amd.geometry_grid(W, H, 2500.0, 0.0, n, n, nz)
+    angles = circular_angles(P)
+    ctx = ct_amd.Context(0)
+    ctx.set_geometry(g)
+    ctx.set_projections(angles)
+    raw_d = torch.empty((P, H, W), dtype=torch.int16, device=gpu)
+    ell = random_ellipsoids(6, 5, 20.0, mu_range=(0.005, 0.02))
+    ctx.phantom_images(ell, 0, P, 0.0, 65535.0, 5, raw_d, None)
+    torch.cuda.synchronize()
+    raw = raw_d.cpu().numpy()
+
+    def recon():
+        out = np.empty(n * n * nz, np.float32)
+        ctx.reconstruct(raw.ctypes.data, "u16", P, 1.0, "cpu", "ramlak", "f32", out.ctypes.data, "zfastest")
+        return out
+
+    assert ctx.plan_reconstruct("u16", P, "f32") == (True, nz)
+    ref = recon()
+    img = W * H
+    work = img * 64 * (4 + 2 * 2) + P * 8
+    slab_col = n * n * 4 * 2
+    # resident, two slabs of 32
+    ctx.set_memory_budget(work + img * 16 * P + slab_col * 32)
+    assert ctx.plan_reconstruct("u16", P, "f32") == (True, 32)
+    got = recon()
+    assert np.array_equal(got, ref)
+    # non-resident: one batch of entries, slabs of 16, re-derived for every slab
+    ctx.set_memory_budget(work + img * 16 * 64 + slab_col * 16 + 100)
+    assert ctx.plan_reconstruct("u16", P, "f32") == (False, 16)
+    got = recon()
+    err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    print(f"non-resident slabs vs resident: rel {err:.2e}")
+    assert err < 1e-6
+    # nothing fits
+    ctx.set_memory_budget(1 << 20)
+    with pytest.raises(_ctg.CtgError) as e:
+        ctx.plan_reconstruct("u16", P, "f32")
+    assert e.value.status == _ctg.ERROR_INSUFFICIENT_VRAM
+    with pytest.raises(_ctg.CtgError) as e:
+        recon()
+    assert e.value.status == _ctg.ERROR_INSUFFICIENT_VRAM
+    # cancellation (CtVolume::stop): reported, and cleared by reset
+    ctx.set_memory_budget(0)
+    ctx.cancel()
+    with pytest.raises(_ctg.CtgError) as e:
+        recon()
+    assert e.value.status == _ctg.ERROR_INTERRUPTED
+    ctx.reset_cancel()
+    assert np.array_equal(recon(), ref)
