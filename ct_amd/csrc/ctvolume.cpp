@@ -12,6 +12,7 @@
 #include <iostream>
 #include <limits>
 #include <sstream>
+#include <stdexcept>
 #include <thread>
 
 #include <unistd.h>
@@ -224,6 +225,40 @@ unsigned CtVolume::getCrossSectionSize() const {
 }
 
 Axis CtVolume::getCrossSectionAxis() const { return crossSectionAxis_; }
+
+// Ct/CtVolume.cpp:391-393 -> Common/Volume.h:577-641 with LEFT_HANDED.  volume_ is
+// Z_FASTEST, element (x, y, z) at (x * yMax + y) * zMax + z.
+Image CtVolume::getVolumeCrossSection(Axis axis, unsigned index) const {
+    Image out;
+    if (volume_.empty() || xMax_ == 0) return out;
+    const unsigned n = axis == Axis::X ? xMax_ : (axis == Axis::Y ? yMax_ : zMax_);
+    if (index >= n) throw std::out_of_range("Index out of bounds.");
+    const size_t ny = yMax_, nz = zMax_;
+    const int rows = axis == Axis::Z ? (int)xMax_ : (int)zMax_;
+    const int cols = axis == Axis::Y ? (int)xMax_ : (int)yMax_;
+    out.width = cols;
+    out.height = rows;
+    out.depth = PixelDepth::F32;
+    out.data.resize((size_t)rows * cols * sizeof(float));
+    float* dst = reinterpret_cast<float*>(out.data.data());
+    for (int r = 0; r < rows; ++r) {
+        float* row = dst + (size_t)r * cols;
+        if (axis == Axis::Z) {
+            // contiguous in y at stride nz: (x = r, y = c, z = index)
+            const float* src = volume_.data() + (size_t)r * ny * nz + index;
+            for (int c = 0; c < cols; ++c) row[c] = src[(size_t)c * nz];
+        } else if (axis == Axis::X) {
+            // (x = index, y = c, z = rows - 1 - r)
+            const float* src = volume_.data() + (size_t)index * ny * nz + (size_t)(rows - 1 - r);
+            for (int c = 0; c < cols; ++c) row[c] = src[(size_t)c * nz];
+        } else {
+            // (x = cols - 1 - c, y = index, z = rows - 1 - r)
+            const float* src = volume_.data() + (size_t)index * nz + (size_t)(rows - 1 - r);
+            for (int c = 0; c < cols; ++c) row[c] = src[(size_t)(cols - 1 - c) * ny * nz];
+        }
+    }
+    return out;
+}
 
 // ------------------------------------------------------------------- setters
 
@@ -836,6 +871,21 @@ int ctv_status(void* v, char* msg, int len) {
         msg[len - 1] = '\0';
     }
     return s.successful ? 0 : (s.userInterrupted ? 1 : 2);
+}
+// getVolumeCrossSection: returns 1 and writes rows x cols floats to out (when cap, in
+// floats, is large enough); 0 when there is no volume; -1 for an index out of range.
+int ctv_cross_section(void* v, int axis, unsigned index, float* out, unsigned long long cap, int* rows, int* cols) {
+    ct::Image im;
+    try {
+        im = static_cast<ct::CtVolume*>(v)->getVolumeCrossSection((ct::Axis)axis, index);
+    } catch (const std::out_of_range&) {
+        return -1;
+    }
+    *rows = im.height;
+    *cols = im.width;
+    if (im.empty()) return 0;
+    if (out && cap >= (unsigned long long)im.width * im.height) std::memcpy(out, im.data.data(), im.data.size());
+    return 1;
 }
 const float* ctv_volume(void* v, unsigned long long* n) {
     const auto& vol = static_cast<ct::CtVolume*>(v)->volumeData();

@@ -83,6 +83,12 @@ public:
     unsigned getCrossSectionIndex() const;
     unsigned getCrossSectionSize() const;
     Axis getCrossSectionAxis() const;
+    // A slice of the reconstructed volume as a 32-bit float image in the left-handed
+    // orientation (Ct/CtVolume.cpp:391-393, Common/Volume.h:577-641): Axis::X gives
+    // rows = z (top row = highest z) by columns = y; Axis::Y gives rows = z by columns =
+    // x, highest x first; Axis::Z gives rows = x by columns = y.  Empty before a
+    // reconstruction; throws std::out_of_range for an index outside the axis.
+    Image getVolumeCrossSection(Axis axis, unsigned index) const;
 
     // setters (Ct/CtVolume.h:97-113)
     void setCrossSectionIndex(unsigned index);

@@ -116,3 +116,13 @@ def test_facade_skip_projections_and_clockwise(gpu, oracle, tmp_path):
     err = rel_rmse(got, ref)
     print(f"facade skip/cw FDK rel RMSE {err:.3e}")
     assert err < 1e-4
+
+    # getVolumeCrossSection, left-handed (Common/Volume.h:600-625): X -> (z descending, y),
+    # Y -> (z descending, x descending), Z -> (x, y)
+    np.testing.assert_array_equal(f.cross_section(0, 5), got[5].T[::-1, :])
+    np.testing.assert_array_equal(f.cross_section(1, ny - 1), got[:, ny - 1, :].T[::-1, ::-1])
+    np.testing.assert_array_equal(f.cross_section(2, nz // 2), got[:, :, nz // 2])
+    for axis, n in enumerate((nx, ny, nz)):
+        assert f.cross_section(axis, n - 1).size == got.size // n
+        with pytest.raises(IndexError):
+            f.cross_section(axis, n)

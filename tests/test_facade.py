@@ -35,6 +35,8 @@ def load_ctv():
     lib.ctv_status.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
     lib.ctv_volume.restype = ctypes.POINTER(ctypes.c_float)
     lib.ctv_volume.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_ulonglong)]
+    lib.ctv_cross_section.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_void_p,
+                                      ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     lib.ctv_decode_image.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_int), ctypes.c_void_p, ctypes.c_ulonglong,
                                      ctypes.c_char_p, ctypes.c_int]
@@ -79,6 +81,20 @@ class Facade:
         n = ctypes.c_ulonglong()
         p = self.lib.ctv_volume(self.h, ctypes.byref(n))
         return np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else np.zeros(0, np.float32)
+
+    def cross_section(self, axis, index):
+        """getVolumeCrossSection(axis 0/1/2 = X/Y/Z, index): None when there is no
+        volume; raises IndexError for an index outside the axis."""
+        rows, cols = ctypes.c_int(), ctypes.c_int()
+        code = self.lib.ctv_cross_section(self.h, axis, index, None, 0, ctypes.byref(rows), ctypes.byref(cols))
+        if code < 0:
+            raise IndexError("Index out of bounds.")
+        if code == 0:
+            return None
+        out = np.empty((rows.value, cols.value), np.float32)
+        self.lib.ctv_cross_section(self.h, axis, index, out.ctypes.data, out.size, ctypes.byref(rows),
+                                   ctypes.byref(cols))
+        return out
 
 
 def decode(lib, path):
@@ -240,3 +256,5 @@ def test_cpu_path_is_not_provided(ctv, tmp_path):
     code, msg = f.status()
     assert code == 2 and "CPU path" in msg
     assert f.volume().size == 0
+    # no volume: an empty cross section for any axis (Common/Volume.h:578)
+    assert f.cross_section(2, 0) is None and f.cross_section(0, 10**6) is None
