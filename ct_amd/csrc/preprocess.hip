@@ -8,12 +8,14 @@
 //    window with r = k/(W/2+1), unnormalised real inverse: Ct/CtVolume.cpp:41-101,
 //    832-858, 910-917 and Ct/cuda_ct.cu:48-100.
 //
-// Design: one workgroup filters two rows at once as the real and imaginary parts of
-// one complex sequence (the response is real and even, so real rows stay real).
-// The whole row pair lives in LDS; a power-of-two W runs a radix-2 DIT forward FFT
-// (bit-reversed load), multiplies by the response and conjugates, then a DIF FFT
-// whose bit-reversed output is read back in natural order.  Other W use a direct
-// circular convolution with the precomputed kernel.
+// Design: two rows are filtered at once as the real and imaginary parts of one complex
+// sequence (the response is real and even, so real rows stay real).
+//  - W = 256..2048 (FP32): one wave per row pair, values in registers, Stockham
+//    radix-16/8/4 passes exchanging through the wave's LDS region (rows_fft_kernel).
+//  - Other powers of two, and the FP64 yardstick: one workgroup per row pair, the pair
+//    in LDS, a radix-2 DIT forward FFT (bit-reversed load), the response and a
+//    conjugation, then a DIF FFT whose bit-reversed output is read back in natural order.
+//  - Other W: a direct circular convolution with the precomputed kernel.
 #include "ctg_internal.h"
 
 #include <hip/hip_fp16.h>
@@ -229,6 +231,270 @@ __global__ __launch_bounds__(kRowThreads) void rows_kernel(PreArgs a, const Tin*
     }
 }
 
+// ---------------------------------------------------------------------------------
+// Register-resident row filter for W = 256, 512, 1024, 2048 (FP32).
+//
+// One wave owns one row pair (rows a and b as the real and imaginary parts), each lane
+// holds P = W/64 complex values, and the transform is a Stockham autosort FFT of 3-4
+// passes with radix 16/8/4 butterflies evaluated in registers; between passes the wave
+// exchanges values through its own LDS region (no workgroup barrier: a wave's LDS
+// operations execute in order).  Pass i, radix R, stride NS: butterfly j reads
+// x[j + r*W/R], multiplies by exp(-2 pi i (j mod NS) r / (NS R)) and writes the R-point
+// DFT to y[(j / NS) NS R + j mod NS + r NS]; the output ends in natural order.  The
+// filtered rows are the conjugate of the forward transform of conj(X * H), so the same
+// forward passes run twice; the first pass reads the raw pixels from global memory and
+// the last writes the filtered rows.
+namespace fftw {
+
+__host__ __device__ constexpr int passes(int N) { return N == 256 ? 4 : 3; }
+__host__ __device__ constexpr int radix(int N, int i) {
+    return N == 256 ? 4 : (N == 512 ? 8 : (i < 2 ? 16 : (N == 1024 ? 4 : 8)));
+}
+__host__ __device__ constexpr int stride(int N, int i) { return i == 0 ? 1 : stride(N, i - 1) * radix(N, i - 1); }
+// LDS position of element i: one float2 of padding per 16 spreads the strided writes
+__device__ inline int lds_pos(int i) { return i + (i >> 4); }
+
+struct F2 {
+    float x, y;
+};
+__device__ inline F2 add(F2 a, F2 b) { return {a.x + b.x, a.y + b.y}; }
+__device__ inline F2 sub(F2 a, F2 b) { return {a.x - b.x, a.y - b.y}; }
+__device__ inline F2 mul(F2 a, F2 w) { return {fmaf(a.x, w.x, -(a.y * w.y)), fmaf(a.x, w.y, a.y * w.x)}; }
+__device__ inline F2 mul_mi(F2 a) { return {a.y, -a.x}; }  // a * (-i)
+
+// in-place forward DFTs (kernel exp(-2 pi i n k / R)), natural order in and out
+__device__ inline void dft4(F2& a0, F2& a1, F2& a2, F2& a3) {
+    const F2 t0 = add(a0, a2), t1 = sub(a0, a2), t2 = add(a1, a3), t3 = mul_mi(sub(a1, a3));
+    a0 = add(t0, t2);
+    a2 = sub(t0, t2);
+    a1 = add(t1, t3);
+    a3 = sub(t1, t3);
+}
+__device__ inline void dft8(F2* a) {
+    constexpr float h = 0.70710678118654752440f;
+    F2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+    F2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+    dft4(e0, e1, e2, e3);
+    dft4(o0, o1, o2, o3);
+    o1 = mul(o1, F2{h, -h});
+    o2 = mul_mi(o2);
+    o3 = mul(o3, F2{-h, -h});
+    a[0] = add(e0, o0); a[4] = sub(e0, o0);
+    a[1] = add(e1, o1); a[5] = sub(e1, o1);
+    a[2] = add(e2, o2); a[6] = sub(e2, o2);
+    a[3] = add(e3, o3); a[7] = sub(e3, o3);
+}
+__device__ inline void dft16(F2* a) {
+    // 16 = 4 x 4: DFT4 over n_a of a[4 n_a + n_b], twiddle exp(-2 pi i n_b k_a / 16),
+    // DFT4 over n_b; X[k_a + 4 k_b]
+    constexpr float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f, h = 0.70710678118654752440f;
+    F2 t[4][4];
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+        t[nb][0] = a[nb]; t[nb][1] = a[4 + nb]; t[nb][2] = a[8 + nb]; t[nb][3] = a[12 + nb];
+        dft4(t[nb][0], t[nb][1], t[nb][2], t[nb][3]);
+    }
+    t[1][1] = mul(t[1][1], F2{c1, -s1});
+    t[1][2] = mul(t[1][2], F2{h, -h});
+    t[1][3] = mul(t[1][3], F2{s1, -c1});
+    t[2][1] = mul(t[2][1], F2{h, -h});
+    t[2][2] = mul_mi(t[2][2]);
+    t[2][3] = mul(t[2][3], F2{-h, -h});
+    t[3][1] = mul(t[3][1], F2{s1, -c1});
+    t[3][2] = mul(t[3][2], F2{-h, -h});
+    t[3][3] = mul(t[3][3], F2{-c1, s1});
+#pragma unroll
+    for (int ka = 0; ka < 4; ++ka) {
+        F2 y0 = t[0][ka], y1 = t[1][ka], y2 = t[2][ka], y3 = t[3][ka];
+        dft4(y0, y1, y2, y3);
+        a[ka] = y0; a[ka + 4] = y1; a[ka + 8] = y2; a[ka + 12] = y3;
+    }
+}
+template <int R>
+__device__ inline void dft(F2* a) {
+    if constexpr (R == 4) dft4(a[0], a[1], a[2], a[3]);
+    else if constexpr (R == 8) dft8(a);
+    else dft16(a);
+}
+
+// exp(-2 pi i k / N) for 0 <= k < N from the half table exp(-2 pi i m / N), m < N/2
+template <int N>
+__device__ inline F2 twiddle(const Cx<float>* __restrict__ tw, int k) {
+    const Cx<float> t = tw[k & (N / 2 - 1)];
+    return (k & (N / 2)) ? F2{-t.re, -t.im} : F2{t.re, t.im};
+}
+
+// orders the wave's LDS stores and loads around a cross-lane exchange
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int N, int I>
+__device__ inline void load_pass(const F2* __restrict__ buf, F2 (&v)[N / 64], int lane) {
+    constexpr int R = radix(N, I), B = N / 64 / R;
+#pragma unroll
+    for (int b = 0; b < B; ++b)
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b * R + r] = buf[lds_pos(lane + 64 * b + r * (N / R))];
+}
+
+template <int N, int I>
+__device__ inline void store_pass(F2* __restrict__ buf, const F2 (&v)[N / 64], int lane) {
+    constexpr int R = radix(N, I), NS = stride(N, I), B = N / 64 / R;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = lane + 64 * b;
+        const int base = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[lds_pos(base + r * NS)] = v[b * R + r];
+    }
+}
+
+// Passes I.. of one forward transform.  On entry v holds pass I's inputs; on return
+// v holds the last pass's outputs: value b*R + r is X[lane + 64 b + r * NS_last].
+template <int N, int I>
+__device__ inline void passes_from(F2 (&v)[N / 64], F2* __restrict__ buf, int lane,
+                                   const Cx<float>* __restrict__ tw) {
+    constexpr int R = radix(N, I), NS = stride(N, I), B = N / 64 / R;
+    // an opaque zero offset per pass: the twiddles are then loaded where they are used
+    // instead of being hoisted and kept live across both transforms
+    int zero = 0;
+    asm volatile("" : "+s"(zero));
+    const Cx<float>* twp = tw + zero;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        if constexpr (NS > 1) {
+            const int jm = (lane + 64 * b) % NS;
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[b * R + r] = mul(v[b * R + r], twiddle<N>(twp, jm * r * (N / (NS * R))));
+        }
+        dft<R>(v + b * R);
+    }
+    if constexpr (I + 1 < passes(N)) {
+        store_pass<N, I>(buf, v, lane);
+        wave_sync();
+        load_pass<N, I + 1>(buf, v, lane);
+        wave_sync();
+        passes_from<N, I + 1>(v, buf, lane, tw);
+    }
+}
+
+}  // namespace fftw
+
+// row pairs (waves) per workgroup; LDS is (N + N/16) float2 per wave
+__host__ __device__ constexpr int fft_waves(int N) { return N >= 2048 ? 2 : 4; }
+
+template <int N, typename Tin>
+__global__ __launch_bounds__(64 * fft_waves(N)) void rows_fft_kernel(PreArgs a, const Tin* __restrict__ raw,
+                                                                  const float* __restrict__ lin,
+                                                                  const float* __restrict__ resp,
+                                                                  const Cx<float>* __restrict__ tw,
+                                                                  float* __restrict__ out,
+                                                                  unsigned* __restrict__ maxabs) {
+    using namespace fftw;
+    constexpr int P = N / 64;
+    constexpr int LAST = passes(N) - 1;
+    constexpr int R0 = radix(N, 0), RL = radix(N, LAST), NSL = stride(N, LAST);
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    F2* buf = reinterpret_cast<F2*>(smem_raw) + (size_t)wave * (N + N / 16);
+    const long long pair = (long long)blockIdx.x * fft_waves(N) + wave;
+    const long long ra = 2 * pair, rb = ra + 1;
+    if (ra >= a.nrows) return;  // waves share nothing, so a finished wave may leave
+    const bool hasb = rb < a.nrows;
+    const int rin_a = (int)(ra % a.H), rin_b = (int)(rb % a.H);
+    const size_t offa = (size_t)ra * N, offb = (size_t)rb * N;
+
+    F2 v[P];
+    // pass 0 inputs straight from the images: x[lane + 64 b + r N/R0].  All loads are
+    // issued first (a missing row b re-reads row a and is zeroed below); the uniform
+    // log and weight flags are tested once, outside the per-pixel loops.
+    const size_t offb_ld = hasb ? offb : offa;
+    if (raw != nullptr) {
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            const int n = lane + 64 * (i / R0) + (i % R0) * (N / R0);
+            v[i].x = (float)raw[offa + n];
+            v[i].y = (float)raw[offb_ld + n];
+        }
+        if (a.do_log) {
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                v[i].x = -logf(fmaf(v[i].x, a.la, a.lb) * a.lk);
+                v[i].y = -logf(fmaf(v[i].y, a.la, a.lb) * a.lk);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            const int n = lane + 64 * (i / R0) + (i % R0) * (N / R0);
+            v[i].x = lin[offa + n];
+            v[i].y = lin[offb_ld + n];
+        }
+    }
+    if (a.do_weight) {
+        // D / sqrt(D^2 + u^2 + v^2), u = column - uPre, v = vPre - row
+        const float D = a.fcd;
+        const float va = (float)(-rin_a) + a.vpre, vb = (float)(-rin_b) + a.vpre;
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            const int n = lane + 64 * (i / R0) + (i % R0) * (N / R0);
+            const float u = (float)n - a.upre;
+            const float du = D * D + u * u;
+            v[i].x = v[i].x * (D / sqrtf(du + va * va));
+            v[i].y = v[i].y * (D / sqrtf(du + vb * vb));
+        }
+    }
+    if (!hasb) {
+#pragma unroll
+        for (int i = 0; i < P; ++i) v[i].y = 0.0f;
+    }
+    passes_from<N, 0>(v, buf, lane, tw);
+    // multiply by the real even response and conjugate; back to pass 0's distribution
+#pragma unroll
+    for (int b = 0; b < P / RL; ++b)
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+            const int k = lane + 64 * b + r * NSL;
+            const float h = resp[k];
+            const F2 z = v[b * RL + r];
+            buf[lds_pos(k)] = F2{z.x * h, -(z.y * h)};
+        }
+    wave_sync();
+    load_pass<N, 0>(buf, v, lane);
+    wave_sync();
+    passes_from<N, 0>(v, buf, lane, tw);
+
+    float mloc = 0.0f, ma = 0.0f, mb = 0.0f;
+#pragma unroll
+    for (int b = 0; b < P / RL; ++b)
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+            const int k = lane + 64 * b + r * NSL;
+            const float ya = v[b * RL + r].x, yb = -v[b * RL + r].y;
+            out[offa + k] = ya;
+            if (hasb) out[offb + k] = yb;
+            ma = fmaxf(ma, fabsf(ya));
+            mb = fmaxf(mb, fabsf(yb));
+        }
+    if (maxabs != nullptr) {
+        const long long pa = ra / a.H, pb = hasb ? rb / a.H : pa;
+        if (pa == pb) {
+            mloc = wave_max(fmaxf(ma, hasb ? mb : 0.0f));
+            if (lane == 0) atomicMax(&maxabs[pa], __float_as_uint(mloc));
+        } else {
+            ma = wave_max(ma);
+            mb = wave_max(mb);
+            if (lane == 0) {
+                atomicMax(&maxabs[pa], __float_as_uint(ma));
+                atomicMax(&maxabs[pb], __float_as_uint(mb));
+            }
+        }
+    }
+}
+
 // Row-major FP32 images -> backprojection layout.  Entry (u, v) of image p holds the
 // bilinear coefficients of the cell whose top-left texel is (u, v):
 //   A = q[v][u], dU = q[v][u+1] - A, dV = q[v+1][u] - A, dUV = q[v+1][u+1] - q[v+1][u] - dU
@@ -315,6 +581,33 @@ void fill_log_params(PreArgs& a, ctg_pixel_type pixel, float base, ctg_log_form 
     a.lk = (float)a.dlk;
 }
 
+template <int N>
+ctg_status launch_rows_fft(const void* raw, ctg_pixel_type pixel, const float* lin, long long nrows,
+                           const PreArgs& a, const float* resp, const Cx<float>* tw, float* out,
+                           unsigned* maxabs, hipStream_t st) {
+    constexpr int waves = fft_waves(N);
+    const long long pairs = (nrows + 1) / 2;
+    const long long blocks = (pairs + waves - 1) / waves;
+    if (blocks > 0x7fffffffll) return fail(CTG_ERROR_UNSUPPORTED, "too many detector rows in one call");
+    const size_t lds = (size_t)waves * (N + N / 16) * sizeof(fftw::F2);
+    const dim3 grid((unsigned)blocks), block(64 * waves);
+    if (raw == nullptr) {
+        hipLaunchKernelGGL((rows_fft_kernel<N, float>), grid, block, lds, st, a, (const float*)nullptr, lin, resp,
+                           tw, out, maxabs);
+    } else if (pixel == CTG_PIXEL_U8) {
+        hipLaunchKernelGGL((rows_fft_kernel<N, uint8_t>), grid, block, lds, st, a, (const uint8_t*)raw, lin, resp,
+                           tw, out, maxabs);
+    } else if (pixel == CTG_PIXEL_U16) {
+        hipLaunchKernelGGL((rows_fft_kernel<N, uint16_t>), grid, block, lds, st, a, (const uint16_t*)raw, lin,
+                           resp, tw, out, maxabs);
+    } else {
+        hipLaunchKernelGGL((rows_fft_kernel<N, float>), grid, block, lds, st, a, (const float*)raw, lin, resp, tw,
+                           out, maxabs);
+    }
+    CTG_HIP(hipGetLastError());
+    return CTG_OK;
+}
+
 template <typename T>
 ctg_status launch_rows(ctg_context* ctx, const void* raw, ctg_pixel_type pixel, const float* lin,
                        long long nrows, const PreArgs& a, T* out, unsigned* maxabs, hipStream_t st) {
@@ -326,6 +619,16 @@ ctg_status launch_rows(ctg_context* ctx, const void* raw, ctg_pixel_type pixel, 
     if (lds > 160 * 1024) return fail(CTG_ERROR_UNSUPPORTED, "detector width %d exceeds the LDS row buffer", a.W);
     if (a.do_filter && !a.pow2 && a.W > 8 * kRowThreads)
         return fail(CTG_ERROR_UNSUPPORTED, "non power-of-two detector width %d > %d", a.W, 8 * kRowThreads);
+    if constexpr (sizeof(T) == 4) {
+        if (a.do_filter && a.pow2 && a.W >= 256 && a.W <= 2048) {
+            switch (a.W) {
+                case 256: return launch_rows_fft<256>(raw, pixel, lin, nrows, a, resp, tw, out, maxabs, st);
+                case 512: return launch_rows_fft<512>(raw, pixel, lin, nrows, a, resp, tw, out, maxabs, st);
+                case 1024: return launch_rows_fft<1024>(raw, pixel, lin, nrows, a, resp, tw, out, maxabs, st);
+                default: return launch_rows_fft<2048>(raw, pixel, lin, nrows, a, resp, tw, out, maxabs, st);
+            }
+        }
+    }
     dim3 grid((unsigned)((nrows + 1) / 2));
     if (raw == nullptr) {
         hipLaunchKernelGGL((rows_kernel<T, float>), grid, dim3(kRowThreads), lds, st, a, (const float*)nullptr,

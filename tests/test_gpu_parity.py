@@ -78,6 +78,50 @@ def test_preprocess_matches_oracle(gpu, oracle):
             assert np.all(scales.cpu().numpy() == 1.0)
 
 
+@pytest.mark.parametrize("W,H,P,pixel", [(256, 7, 3, "u16"), (512, 5, 3, "u8"), (1024, 9, 2, "u16"),
+                                         (2048, 3, 3, "u16")])
+def test_register_fft_widths_match_oracle(gpu, oracle, W, H, P, pixel):
+    """Widths 256..2048 take the register-resident row filter.  An odd row count puts a
+    row pair across two images and leaves the last pair without its second row."""
+    import torch
+    import ct_amd
+    rng = np.random.default_rng(W)
+    hi = 255 if pixel == "u8" else 65535
+    dt = np.uint8 if pixel == "u8" else np.uint16
+    raw = rng.integers(hi // 20, hi, size=(P, H, W)).astype(dt)
+    g, _ = ct_amd.geometry_ctvolume(W, H, 3.0 * W, 0.0, None)
+    ctx = ct_amd.Context(0)
+    ctx.set_geometry(g)
+    ctx.set_projections(np.linspace(0.0, 360.0, P, endpoint=False), np.zeros(P))
+    base = float(hi)
+    for filt, log_form in (("ramlak", "cpu"), ("hann", "gpu"), ("shepplogan", "cpu")):
+        filtered = torch.empty((P, W, H, 4), dtype=torch.float32, device="cuda")
+        scales = torch.empty(P, dtype=torch.float32, device="cuda")
+        ctx.preprocess(torch.from_numpy(raw).cuda(), pixel, P, base, log_form, filt, "f32", filtered, scales)
+        torch.cuda.synchronize()
+        got = entries_to_rows(filtered.cpu().numpy())
+        ref = oracle.preprocess_f32(_oracle_geom(g), raw, pixel, base, filt, log_form)
+        err = rel_rmse(got, ref)
+        print(f"W={W} {filt}/{log_form}: rel RMSE {err:.3e}")
+        # FP32 transforms against the oracle's FP64 ones: on these noise rows the filter
+        # removes a large mean, so the error relative to the output is a few 1e-6; a
+        # float32 emulation measures 1.7e-6 at W=1024 for radix-2 and Stockham alike
+        assert err < 5e-6, (W, filt, err)
+        # FP16 entries: per-image power-of-two scale above the image's max |value|
+        f16 = torch.empty(ctx.filtered_bytes("f16", P), dtype=torch.uint8, device="cuda")
+        ctx.preprocess(torch.from_numpy(raw).cuda(), pixel, P, base, log_form, filt, "f16", f16, scales)
+        torch.cuda.synchronize()
+        m = np.abs(got).reshape(P, -1).max(axis=1)
+        np.testing.assert_array_equal(scales.cpu().numpy(), np.ldexp(1.0, np.frexp(m)[1]).astype(np.float32))
+    # the filter alone on float rows (ctg_filter_rows)
+    rows = rng.standard_normal((P * H, W)).astype(np.float32)
+    out = torch.empty_like(torch.from_numpy(rows)).cuda()
+    ctx.filter_rows(torch.from_numpy(rows).cuda(), out, P * H, "ramlak")
+    torch.cuda.synchronize()
+    ref = oracle.filter_rows(rows, "ramlak")
+    assert rel_rmse(out.cpu().numpy(), ref) < 2e-6
+
+
 def test_backproject_matches_oracle(gpu, oracle):
     import torch
     scan = small_scan(P=30)
