@@ -126,6 +126,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E::kMi
     const T* __restrict__ proj = reinterpret_cast<const T*>(proj_raw);
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int4 boxes[2][64];
+    // per projection: sin, cos, height, and the entry scale in w (written with the boxes)
+    __shared__ float4 params[2][64];
 
     // XCD-aware numbering: dispatch sends workgroup L to XCD L%8; give each XCD a
     // contiguous range of work items.
@@ -329,7 +331,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E::kMi
         // Buffer p&1 holds projection p while it is computed.  The box of p+1 is always
         // produced at least one barrier before it is staged (batches are computed one
         // batch ahead by wave 0).
-        if (tid < 64) boxes[0][tid] = tid < a.nproj ? box_for(tid) : make_int4(0, 0, 0, 0);
+        // wave 0 fills one batch of 64 projections: footprint boxes and parameters
+        auto fill_batch = [&](int slot, int pn) {
+            boxes[slot][tid] = pn < a.nproj ? box_for(pn) : make_int4(0, 0, 0, 0);
+            float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (pn < a.nproj) {
+                g = pp[pn];
+                g.w = scales ? scales[pn] : 1.0f;
+            }
+            params[slot][tid] = g;
+        };
+        if (tid < 64) fill_batch(0, tid);
         __syncthreads();
         auto buffer = [&](int p) { return smem + (size_t)(p & 1) * a.cap * E::kBytes; };
         if (a.nproj > 0) {
@@ -349,25 +361,27 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E::kMi
             return make_int4(__builtin_amdgcn_readfirstlane(b.x), __builtin_amdgcn_readfirstlane(b.y),
                              __builtin_amdgcn_readfirstlane(b.z), __builtin_amdgcn_readfirstlane(b.w));
         };
+        auto uniform_f = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
+        auto uniform_f4 = [&](float4 v) {
+            return make_float4(uniform_f(v.x), uniform_f(v.y), uniform_f(v.z), uniform_f(v.w));
+        };
+        // The box and parameters of projection p+1 are read from LDS during step p, so a
+        // step starts without waiting for memory other than its own staging DMA.
+        int4 bx = uniform_box(boxes[0][0]);
+        float4 g = uniform_f4(params[0][0]);
         for (int p = 0; p < a.nproj; ++p) {
-            const int4 bx = uniform_box(boxes[(p >> 6) & 1][p & 63]);
-            // projection parameters are loaded before the wait below, which covers them:
-            // a global load issued after the DMA of p+1 would make its first use wait for
-            // that DMA too (the memory counter retires in order)
-            const float4 g = pp[p];
-            const float sc = scales ? scales[p] : 1.0f;
             // this wave's DMA for projection p has landed; the barrier publishes every
             // wave's and guarantees that nobody still reads buffer (p+1)&1 or the box
             // batch about to be overwritten
             __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
             __syncthreads();
-            if ((p & 63) == 0 && tid < 64) {
-                const int pn = p + 64 + tid;
-                boxes[((p >> 6) + 1) & 1][tid] = pn < a.nproj ? box_for(pn) : make_int4(0, 0, 0, 0);
-            }
-            const int4 bn = p + 1 < a.nproj ? uniform_box(boxes[((p + 1) >> 6) & 1][(p + 1) & 63])
-                                            : make_int4(0, 0, 0, 0);
-            step(reinterpret_cast<const T*>(buffer(p)), buffer(p + 1), p, bx, bn, g, sc);
+            if ((p & 63) == 0 && tid < 64) fill_batch(((p >> 6) + 1) & 1, p + 64 + tid);
+            const int q = p + 1;
+            const int4 bn = q < a.nproj ? uniform_box(boxes[(q >> 6) & 1][q & 63]) : make_int4(0, 0, 0, 0);
+            const float4 gn = q < a.nproj ? uniform_f4(params[(q >> 6) & 1][q & 63]) : g;
+            step(reinterpret_cast<const T*>(buffer(p)), buffer(p + 1), p, bx, bn, g, g.w);
+            bx = bn;
+            g = gn;
         }
     }
 
