@@ -85,19 +85,30 @@ struct EBF16 {
     }
 };
 
-// Global->LDS DMA of `bytes` (4 or 16) per lane into lds + lane*bytes.  The builtin is
-// only visible to the device pass (the host pass rejects the gfx950-only 16-byte form
-// and would silently drop the kernel instantiation).
-__device__ inline void lds_dma16(const void* gsrc, void* lds_wave_base) {
+// Global->LDS DMA through a buffer resource: B bytes (4 or 16) per lane from byte
+// offset soff + voff of the resource into lds + lane*B.  The buffer form matters: an
+// outstanding global_load_lds is modelled by the compiler as a flat access to memory
+// and LDS, after which it waits for every outstanding LDS read (lgkmcnt(0)) at each LDS
+// use and the read pipeline collapses; a buffer load to LDS is tracked by the memory
+// counter only.  Offsets past the resource's size read zeros.  The builtins exist only
+// in the device pass.
 #if defined(__HIP_DEVICE_COMPILE__)
-    __builtin_amdgcn_global_load_lds(gsrc, lds_wave_base, 16, 0, 0);
-#endif
+using dma_rsrc = __amdgpu_buffer_rsrc_t;
+__device__ inline dma_rsrc make_dma_rsrc(const void* base, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, bytes, 0x00020000);
 }
-__device__ inline void lds_dma4(const void* gsrc, void* lds_wave_base) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    __builtin_amdgcn_global_load_lds(gsrc, lds_wave_base, 4, 0, 0);
-#endif
+template <int B>
+__device__ inline void lds_dma(dma_rsrc rs, unsigned voff, int soff, void* lds_wave_base) {
+    auto* l = (__attribute__((address_space(3))) void*)lds_wave_base;
+    if constexpr (B == 16) __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, l, 16, voff, soff, 0, 0);
+    else __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, l, 4, voff, soff, 0, 0);
 }
+#else
+struct dma_rsrc {};
+__device__ inline dma_rsrc make_dma_rsrc(const void*, int) { return {}; }
+template <int B>
+__device__ inline void lds_dma(dma_rsrc, unsigned, int, void*) {}
+#endif
 
 __device__ inline int cvt_flr(float x) {
     int r;
@@ -229,19 +240,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E::kMi
             constexpr bool wide = DMA == 16;
             constexpr int ub = DMA;  // bytes per lane per DMA
             const int units = wide ? (nr * E::kBytes + 15) / 16 : nr * (E::kBytes / 4);
-            const char* src = reinterpret_cast<const char*>(proj) +
-                              (((size_t)p * a.W + us) * a.H + vs) * E::kBytes;
+            // one resource per projection image (its size is below 2^31 bytes, checked on
+            // the host); the column offsets are wave-uniform
+            const int img_bytes = a.W * a.H * E::kBytes;
+            const dma_rsrc rs = make_dma_rsrc(reinterpret_cast<const char*>(proj) + (size_t)p * img_bytes, img_bytes);
+            const int col0 = (us * a.H + vs) * E::kBytes;
             const unsigned lane_off = (unsigned)lane * ub;
             for (int c = wave_u; c < nc; c += kWaves) {
-                const char* gcol = src + (size_t)c * a.H * E::kBytes;
+                const int gcol = col0 + c * a.H * E::kBytes;
                 unsigned char* lcol = dst + (size_t)c * pitch * E::kBytes;
                 for (int j = 0; j < units; j += 64) {
-                    if (lane < units - j) {
-                        const void* g = gcol + (size_t)j * ub + lane_off;
-                        void* l = (void*)(lcol + (size_t)j * ub);
-                        if constexpr (wide) lds_dma16(g, l);
-                        else lds_dma4(g, l);
-                    }
+                    if (lane < units - j) lds_dma<ub>(rs, lane_off, gcol + j * ub, lcol + (size_t)j * ub);
                 }
             }
         };
@@ -355,6 +364,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E::kMi
         auto step = [&](const T* __restrict__ cur, unsigned char* __restrict__ nxt, int p, int4 bx, int4 bn,
                         float4 g, float sc) {
             if (box_kind(bn) == 1) stage(nxt, p + 1, bn);
+            __builtin_amdgcn_sched_barrier(0);  // the DMA is issued before the compute
             compute(cur, p, bx, g, sc);
         };
         auto uniform_box = [](int4 b) {
