@@ -178,10 +178,12 @@ def main():
             f"({updates / bp / 1e9:.1f} GUPS)")
         return t, bp, filtered, scales
 
-    def e2e_from_pinned_host(store, chunks=8):
+    def e2e_from_pinned_host(store, chunks=8, groups=4):
         """H2D of the raw uint16 projections from pinned host memory in chunks on an upload
-        stream, each chunk preprocessed on the compute stream as soon as it lands, then
-        the backprojection: the whole reconstruction from host images to the volume in HBM."""
+        stream; each chunk is preprocessed on the compute stream as soon as it lands, and
+        each group of chunks is backprojected (accumulating) as soon as it is
+        preprocessed, so later uploads run under earlier backprojections: the whole
+        reconstruction from host images to the volume in HBM."""
         filtered, scales = make_bufs(store)
         raw16 = raw.view(torch.int16)  # same bytes; int16 has the full set of copy kernels
         host = torch.empty(raw.shape, dtype=torch.int16, pin_memory=True)
@@ -202,12 +204,16 @@ def main():
                     e = torch.cuda.Event()
                     e.record(up)
                     evs.append(e)
-            for i in range(chunks):
-                a, b = bounds[i], bounds[i + 1]
-                cmp.wait_event(evs[i])
-                ctx.preprocess(dev_raw[a:b], "u16", b - a, base, "cpu", filt, store, filtered[a * per:b * per],
-                               scales[a:b], cmp.cuda_stream)
-            ctx.backproject(filtered, store, scales, 0, P, vol, 0, g.nz, False, cmp.cuda_stream)
+            for grp in range(groups):
+                c0, c1 = chunks * grp // groups, chunks * (grp + 1) // groups
+                for i in range(c0, c1):
+                    a, b = bounds[i], bounds[i + 1]
+                    cmp.wait_event(evs[i])
+                    ctx.preprocess(dev_raw[a:b], "u16", b - a, base, "cpu", filt, store, filtered[a * per:b * per],
+                                   scales[a:b], cmp.cuda_stream)
+                a, b = bounds[c0], bounds[c1]
+                ctx.backproject(filtered[a * per:b * per], store, scales[a:b], a, b - a, vol, 0, g.nz, grp > 0,
+                                cmp.cuda_stream)
 
         run()
         torch.cuda.synchronize()
@@ -229,6 +235,20 @@ def main():
         del host, dev_raw, filtered, scales
         return t, th2d
 
+    def reconstruct_host_to_host():
+        """ctg_reconstruct, the C-ABI pipeline behind the CtVolume facade: pageable host
+        uint16 images in, Z_FASTEST host FP32 volume out (one warm-up and one timed run)."""
+        import numpy as np
+        host_raw = raw.view(torch.int16).cpu().numpy().view(np.uint16)
+        out_vol = np.empty(g.nx * g.ny * g.nz, np.float32)
+        ctx.reconstruct(host_raw.ctypes.data, "u16", P, base, "cpu", filt, "f32", out_vol.ctypes.data, "zfastest")
+        t0 = time.perf_counter()
+        ctx.reconstruct(host_raw.ctypes.data, "u16", P, base, "cpu", filt, "f32", out_vol.ctypes.data, "zfastest")
+        t = time.perf_counter() - t0
+        log(f"ctg_reconstruct host to host: {t * 1e3:.1f} ms")
+        del host_raw, out_vol
+        return t
+
     res = {}
     if args.only == "f16":
         t16, bp16, _, _ = run_store("f16")
@@ -237,6 +257,7 @@ def main():
     t32, bp32, filt32, sc32 = run_store("f32")
     res["f32"] = (t32, bp32)
     e2e = None if args.no_e2e else e2e_from_pinned_host("f32")
+    h2h = None if args.no_e2e else reconstruct_host_to_host()
     vol32 = vol.clone() if not args.no_precision else None
     rows_for_cpu = None
     if not args.no_cpu_baseline:
@@ -311,6 +332,7 @@ def main():
                     "f16_over_f32_bp": bp32 / bp16, "f16_over_f32_step": t32 / t16})
     if e2e is not None:
         out["ms_e2e_from_pinned_host_f32"] = e2e[0] * 1e3
+        out["ms_reconstruct_host_to_host_f32"] = h2h * 1e3
         out["ms_h2d_raw"] = e2e[1] * 1e3
     if not args.no_roi:
         out.update(roi_run(ctx, torch, raw, P, base, filt, stream, args.steps))

@@ -7,6 +7,8 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <thread>
+#include <vector>
 
 namespace ctg {
 
@@ -142,6 +144,42 @@ ctg_status preprocess_batched(ctg_context* ctx, const void* d_raw, ctg_pixel_typ
                               d_scales ? d_scales + b0 : nullptr, st));
     }
     return CTG_OK;
+}
+
+// Host copy split over worker threads (a single thread copies pageable memory at a
+// fraction of what the pinned upload that follows can take).
+void parallel_memcpy(void* dst, const void* src, size_t n) {
+    const size_t chunk = (size_t)32 << 20;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t workers = std::min<size_t>(std::min<size_t>(8, hw), (n + chunk - 1) / chunk);
+    if (workers <= 1) {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    const size_t per = (n + workers - 1) / workers;
+    std::vector<std::thread> pool;
+    for (size_t w = 1; w < workers; ++w) {
+        const size_t o = w * per;
+        if (o >= n) break;
+        pool.emplace_back([=] { std::memcpy((char*)dst + o, (const char*)src + o, std::min(per, n - o)); });
+    }
+    std::memcpy(dst, src, std::min(per, n));
+    for (auto& t : pool) t.join();
+}
+
+// Runs fn(i0, i1) over [0, n) split across up to 8 host threads.
+template <class F>
+void parallel_ranges(size_t n, size_t min_per_thread, F fn) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t workers = std::max<size_t>(1, std::min<size_t>(std::min<size_t>(8, hw), n / std::max<size_t>(1, min_per_thread)));
+    const size_t per = (n + workers - 1) / workers;
+    std::vector<std::thread> pool;
+    for (size_t w = 1; w < workers; ++w) {
+        const size_t a = w * per, b = std::min(n, a + per);
+        if (a < b) pool.emplace_back([=, &fn] { fn(a, b); });
+    }
+    fn(0, std::min(n, per));
+    for (auto& t : pool) t.join();
 }
 
 }  // namespace
@@ -585,21 +623,30 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
     const int slab_nz = plan.slab_nz;
     const long long filt = (long long)ctg_filtered_bytes(&g, store, nproj);
 
-    hipStream_t s_up = nullptr, s_cmp = nullptr;
-    hipEvent_t ev_up[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr};
+    // Streams: uploads (s_up), preprocessing (s_pre), backprojection and downloads
+    // (s_cmp).  Staging buffer i (pinned host and device raw) carries batch b with
+    // i = b & 1: the host refills pinned[i] once upload b-2 is done, the upload of b
+    // waits until preprocessing b-2 has read raw_stage[i].
+    hipStream_t s_up = nullptr, s_pre = nullptr, s_cmp = nullptr;
+    hipEvent_t ev_up[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr}, ev_dl[2] = {nullptr, nullptr},
+               ev_bp = nullptr;
     void* pinned[2] = {nullptr, nullptr};
     float* d_slab = nullptr;
     void* d_conv = nullptr;
     ctg_status st = CTG_OK;
     auto cleanup = [&]() {
         if (s_up) (void)hipStreamSynchronize(s_up);
+        if (s_pre) (void)hipStreamSynchronize(s_pre);
         if (s_cmp) (void)hipStreamSynchronize(s_cmp);
         for (int i = 0; i < 2; ++i) {
             if (ev_up[i]) (void)hipEventDestroy(ev_up[i]);
             if (ev_done[i]) (void)hipEventDestroy(ev_done[i]);
+            if (ev_dl[i]) (void)hipEventDestroy(ev_dl[i]);
             if (pinned[i]) (void)hipHostFree(pinned[i]);
         }
+        if (ev_bp) (void)hipEventDestroy(ev_bp);
         if (s_up) (void)hipStreamDestroy(s_up);
+        if (s_pre) (void)hipStreamDestroy(s_pre);
         if (s_cmp) (void)hipStreamDestroy(s_cmp);
         if (d_slab) (void)hipFree(d_slab);
         if (d_conv) (void)hipFree(d_conv);
@@ -623,12 +670,17 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
         }                         \
     } while (0)
 
+    // pinned staging: one batch of raw images, and at least 16 MiB for the downloads
+    const size_t stage_bytes = std::max(img * pb * batch, (size_t)16 << 20);
     PIPE_HIP(hipStreamCreateWithFlags(&s_up, hipStreamNonBlocking));
+    PIPE_HIP(hipStreamCreateWithFlags(&s_pre, hipStreamNonBlocking));
     PIPE_HIP(hipStreamCreateWithFlags(&s_cmp, hipStreamNonBlocking));
+    PIPE_HIP(hipEventCreateWithFlags(&ev_bp, hipEventDisableTiming));
     for (int i = 0; i < 2; ++i) {
         PIPE_HIP(hipEventCreateWithFlags(&ev_up[i], hipEventDisableTiming));
         PIPE_HIP(hipEventCreateWithFlags(&ev_done[i], hipEventDisableTiming));
-        PIPE_HIP(hipHostMalloc(&pinned[i], img * pb * batch, hipHostMallocDefault));
+        PIPE_HIP(hipEventCreateWithFlags(&ev_dl[i], hipEventDisableTiming));
+        PIPE_HIP(hipHostMalloc(&pinned[i], stage_bytes, hipHostMallocDefault));
     }
     PIPE_TRY(ctx->raw_stage.reserve(img * pb * batch * 2));
     // resident: every projection's entries stay in HBM for all slabs; otherwise one
@@ -639,44 +691,88 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
     if (order == CTG_X_FASTEST) PIPE_HIP(hipMalloc(&d_conv, col * 4 * (size_t)slab_nz));
     const int nb_total = (nproj + batch - 1) / batch;
 
-    // Upload batch b through pinned staging (double-buffered on the upload stream) and
-    // preprocess it on the compute stream into `out` (entries of projections b0..).
+    // Upload batch b through pinned staging and preprocess it on s_pre into `out`
+    // (entries of projections b*batch..).  Returns after the host copy; the GPU work is
+    // queued.  ev_done[b & 1] marks the end of its preprocessing.
     auto stage_batch = [&](int b, char* out) -> ctg_status {
         if (ctx->cancel.load()) return fail(CTG_ERROR_INTERRUPTED, "interrupted");
         const int i = b & 1;
         const int b0 = b * batch, nb = std::min(batch, nproj - b0);
-        // staging buffer i is free once the preprocessing that read it has finished
-        CTG_HIP(hipEventSynchronize(ev_done[i]));
-        memcpy(pinned[i], (const char*)h_raw + (size_t)b0 * img * pb, (size_t)nb * img * pb);
+        CTG_HIP(hipEventSynchronize(ev_up[i]));  // upload b-2 has read pinned[i]
+        parallel_memcpy(pinned[i], (const char*)h_raw + (size_t)b0 * img * pb, (size_t)nb * img * pb);
         char* dst = (char*)ctx->raw_stage.ptr + (size_t)i * img * pb * batch;
+        CTG_HIP(hipStreamWaitEvent(s_up, ev_done[i], 0));  // preprocessing b-2 has read raw_stage[i]
         CTG_HIP(hipMemcpyAsync(dst, pinned[i], (size_t)nb * img * pb, hipMemcpyHostToDevice, s_up));
         CTG_HIP(hipEventRecord(ev_up[i], s_up));
-        CTG_HIP(hipStreamWaitEvent(s_cmp, ev_up[i], 0));
+        CTG_HIP(hipStreamWaitEvent(s_pre, ev_up[i], 0));
         CTG_TRY(preprocess_batched(ctx, dst, pixel, nb, base_intensity, log_form, filter, store, out,
-                                   ctx->scales_tmp.as<float>() + b0, s_cmp));
-        CTG_HIP(hipEventRecord(ev_done[i], s_cmp));
+                                   ctx->scales_tmp.as<float>() + b0, s_pre));
+        CTG_HIP(hipEventRecord(ev_done[i], s_pre));
         return CTG_OK;
     };
-    auto backproject = [&](int p0, int np, int z0, int nz, int accumulate) -> ctg_status {
+    // backprojection on s_cmp of projections [p0, p0+np) once their preprocessing (in
+    // stream order on s_pre, up to the last recorded batch) has finished
+    auto backproject = [&](int p0, int np, int z0, int nz, int accumulate, int last_batch) -> ctg_status {
+        if (last_batch >= 0) CTG_HIP(hipStreamWaitEvent(s_cmp, ev_done[last_batch & 1], 0));
         BpLaunch L{};
         L.g = &g;
         L.d_params = ctx->proj_dev.as<float4>() + p0;
         L.h_params = ctx->proj_host.data() + p0;
         L.d_scales = ctx->scales_tmp.as<float>() + p0;
-        L.d_filtered = ctx->filtered.ptr;
+        L.d_filtered = (const char*)ctx->filtered.ptr + (resident ? (size_t)p0 * img * store_bytes(store) : 0);
         L.store = store;
         L.nproj = np;
         L.d_volume = d_slab;
         L.z0 = z0;
         L.nz = nz;
         L.accumulate = accumulate;
-        return launch_backproject(L, s_cmp);
+        CTG_TRY(launch_backproject(L, s_cmp));
+        CTG_HIP(hipEventRecord(ev_bp, s_cmp));
+        return CTG_OK;
+    };
+    // Slab -> host volume through the pinned staging buffers in chunks: the DMA of chunk
+    // k+1 runs while host threads place chunk k (contiguous for X_FASTEST; for
+    // Z_FASTEST, columns of nz values at a stride of nz_total).  Every earlier GPU step
+    // has finished (the first DMA follows the slab's backprojection on s_cmp, and the
+    // uploads that used the staging buffers precede it).
+    auto download = [&](int z0, int nz) -> ctg_status {
+        const float* src = d_slab;
+        if (order == CTG_X_FASTEST) {
+            CTG_TRY(launch_convert_volume(g, d_slab, nz, CTG_X_FASTEST, CTG_LITTLE_ENDIAN, d_conv, s_cmp));
+            src = (const float*)d_conv;
+        }
+        CTG_HIP(hipStreamSynchronize(s_cmp));
+        const bool zf = order == CTG_Z_FASTEST;
+        const size_t unit = zf ? (size_t)nz : 1;  // floats per unit: a column, or one value
+        const size_t n_units = zf ? col : col * (size_t)nz;
+        const size_t per_chunk = std::max<size_t>(1, stage_bytes / (unit * 4));
+        const size_t nchunks = (n_units + per_chunk - 1) / per_chunk;
+        auto issue = [&](size_t k) -> ctg_status {
+            const size_t u0 = k * per_chunk, u1 = std::min(n_units, u0 + per_chunk);
+            CTG_HIP(hipMemcpyAsync(pinned[k & 1], src + u0 * unit, (u1 - u0) * unit * 4, hipMemcpyDeviceToHost, s_cmp));
+            CTG_HIP(hipEventRecord(ev_dl[k & 1], s_cmp));
+            return CTG_OK;
+        };
+        CTG_TRY(issue(0));
+        for (size_t k = 0; k < nchunks; ++k) {
+            if (k + 1 < nchunks) CTG_TRY(issue(k + 1));
+            CTG_HIP(hipEventSynchronize(ev_dl[k & 1]));
+            const size_t u0 = k * per_chunk, u1 = std::min(n_units, u0 + per_chunk);
+            const float* stage = (const float*)pinned[k & 1];
+            if (!zf) {
+                parallel_memcpy(h_volume + (size_t)z0 * col + u0, stage, (u1 - u0) * 4);
+            } else if (nz == g.nz) {
+                parallel_memcpy(h_volume + u0 * (size_t)g.nz, stage, (u1 - u0) * unit * 4);
+            } else {
+                parallel_ranges(u1 - u0, 4096, [&](size_t a, size_t b) {
+                    for (size_t c = a; c < b; ++c)
+                        std::memcpy(h_volume + (u0 + c) * (size_t)g.nz + z0, stage + c * unit, unit * 4);
+                });
+            }
+        }
+        return CTG_OK;
     };
 
-    if (resident) {
-        for (int b = 0; b < nb_total; ++b)
-            PIPE_TRY(stage_batch(b, (char*)ctx->filtered.ptr + (size_t)b * batch * img * store_bytes(store)));
-    }
     const int nslabs = (g.nz + slab_nz - 1) / slab_nz;
     for (int s = 0; s < nslabs; ++s) {
         if (ctx->cancel.load()) {
@@ -686,26 +782,33 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
         }
         const int z0 = s * slab_nz, nz = std::min(slab_nz, g.nz - z0);
         if (resident) {
-            PIPE_TRY(backproject(0, nproj, z0, nz, 0));
+            // Every slab is backprojected in the same groups of batches (up to four), so
+            // each voxel sees the same sequence of launches whatever the slab plan.  In
+            // the first slab a group is backprojected as soon as it is preprocessed, and
+            // the uploads and preprocessing of the next group run under it.
+            const int groups = std::min(nb_total, 4);
+            int bdone = 0;
+            for (int grp = 0; grp < groups; ++grp) {
+                const int bend = (int)((long long)nb_total * (grp + 1) / groups);
+                if (s == 0) {
+                    for (int b = bdone; b < bend; ++b)
+                        PIPE_TRY(stage_batch(b, (char*)ctx->filtered.ptr + (size_t)b * batch * img * store_bytes(store)));
+                }
+                const int p0 = bdone * batch, p1 = std::min(nproj, bend * batch);
+                PIPE_TRY(backproject(p0, p1 - p0, z0, nz, grp > 0 ? 1 : 0, s == 0 ? bend - 1 : -1));
+                bdone = bend;
+            }
         } else {
             // the single batch buffer is rewritten only after the previous batch's
-            // backprojection (both on the compute stream)
+            // backprojection has read it
             for (int b = 0; b < nb_total; ++b) {
                 const int b0 = b * batch, nb = std::min(batch, nproj - b0);
+                PIPE_HIP(hipStreamWaitEvent(s_pre, ev_bp, 0));
                 PIPE_TRY(stage_batch(b, (char*)ctx->filtered.ptr));
-                PIPE_TRY(backproject(b0, nb, z0, nz, b > 0 ? 1 : 0));
+                PIPE_TRY(backproject(b0, nb, z0, nz, b > 0 ? 1 : 0, b));
             }
         }
-        if (order == CTG_X_FASTEST) {
-            PIPE_TRY(launch_convert_volume(g, d_slab, nz, CTG_X_FASTEST, CTG_LITTLE_ENDIAN, d_conv, s_cmp));
-            PIPE_HIP(hipMemcpyAsync(h_volume + (size_t)z0 * col, d_conv, col * 4 * (size_t)nz, hipMemcpyDeviceToHost,
-                                    s_cmp));
-        } else {
-            // [x][y][z0:z0+nz] of every column: a strided copy into the Z_FASTEST host volume
-            PIPE_HIP(hipMemcpy2DAsync(h_volume + z0, (size_t)g.nz * 4, d_slab, (size_t)nz * 4, (size_t)nz * 4, col,
-                                      hipMemcpyDeviceToHost, s_cmp));
-        }
-        PIPE_HIP(hipStreamSynchronize(s_cmp));
+        PIPE_TRY(download(z0, nz));
         if (progress) progress((double)(s + 1) / nslabs, user);
     }
     cleanup();
