@@ -21,28 +21,43 @@ struct FpArgs {
     float r2, fcd, uoff, ulo, uhi, vlo, vhi;
 };
 
-// Gather form of the projector.  No atomics at all: scattering splats costs LDS float
-// atomics (about 30x an LDS read here) or global ones that contend across workgroups.
-// A workgroup owns one detector column c of one projection and kGRows rows, two
-// adjacent pixels (r, r + 1) per lane, accumulated in registers.
-//  - All lanes walk the same voxel columns: exactly those whose ray lands within
+// Gather form of the projector.  No global or LDS float atomics: scattering splats
+// costs LDS float atomics (about 30x an LDS read here) or global ones that contend
+// across workgroups.  A workgroup owns one detector column c of one projection.
+//  - The voxel columns that can feed column c are exactly those whose ray lands within
 //    [c - 1, c + 1) on the detector (u0 = c takes weight 1 - fu, u0 = c - 1 takes fu).
 //    Along a line of constant outer coordinate u is monotonic in the inner one, so the
 //    two rays from the source through u = c - 1 and u = c + 1 (plus 0.05 pixel) bound
 //    the candidates up to rounding; a 0.01-voxel margin covers that.
-//  - For a voxel column, a lane tests the z whose detector row index lies in
-//    [r - 1, r + 2) (about 3 / U of them) with the backprojection's own arithmetic;
-//    v0 = r - 1, r, r + 1 feed rows r and r + 1 with the bilinear row weights.
-// Every (voxel, tap) pair is therefore added exactly once, by the lane that owns the
-// tap's pixel, and the result does not depend on scheduling.
-constexpr int kGThreads = 256, kGRows = 2 * kGThreads;
+//  - The per-column work (cylinder and window tests, U, the tap weight) is done once per
+//    candidate: the lanes fill a compacted list of candidates in LDS, in walk order.
+//  - Each wave then takes 64 consecutive z of a candidate, one per lane, with the
+//    backprojection's own arithmetic (masks, anchored row index), and adds the two row
+//    taps of its voxel into the wave's own row accumulators in LDS.  Rows of lanes k
+//    apart differ by at least k*U - 1, so lanes are split into k = ceil(2/U) residue
+//    passes whose taps never share a row; the read-modify-writes need no atomics.
+//  - At the end the four waves' rows are summed into the image.
+// Every (voxel, tap) pair is added exactly once, and the order of all additions is fixed:
+// the result does not depend on scheduling.
+constexpr int kGThreads = 256, kGWaves = kGThreads / 64;
+constexpr int kSlots = 1024;  // candidate slots per pass (16 B each)
+constexpr int kMaxRowsFp = 4096;  // detector rows held in LDS per wave
 
 __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const float4* __restrict__ pp,
                                                               const float* __restrict__ vol,
                                                               float* __restrict__ proj) {
+    __shared__ int4 slots[kSlots];
+    __shared__ int wave_cnt[kGWaves];
+    extern __shared__ float rows_lds[];  // kGWaves x (H + 4): row r at index r + 2; rows -2, -1, H, H + 1 absorb taps off the image
     const int c = blockIdx.x;
-    const int r0 = blockIdx.y * kGRows + 2 * (int)threadIdx.x;
-    const int p = blockIdx.z;
+    const int p = blockIdx.y;
+    const int tid = (int)threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int H = a.H, pitch = a.H + 4;
+    float* myrows = rows_lds + (size_t)wv * pitch;
+    for (int i = tid; i < kGWaves * pitch; i += kGThreads) rows_lds[i] = 0.0f;
+    __syncthreads();
+
     const float4 g = pp[p];
     const float sn = g.x, cs = g.y, h = g.z;
     // source and detector points in world xy (voxel units; the virtual detector passes
@@ -61,67 +76,131 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
     const float R = sqrtf(a.r2);
     const int o_lo = max(0, (int)floorf(pre_out - R) - 1), o_hi = min(n_out - 1, (int)ceilf(pre_out + R) + 1);
     const int zend = a.z0 + a.nzs - 1;  // last grid z index of the slab
-    const float rf = (float)r0;
 
-    float acc0 = 0.0f, acc1 = 0.0f;
-    for (int o = o_lo; o <= o_hi; ++o) {
+    // lateral candidates of depth step o: [jl, jh]
+    auto jrange = [&](int o, int& jl, int& jh) {
         const float ow = (float)o - pre_out;
         const float i0 = Si + (ow - So) * k0, i1 = Si + (ow - So) * k1;
-        const int j_lo = max(0, (int)ceilf(fminf(i0, i1) + pre_in - 0.01f));
-        const int j_hi = min(n_in - 1, (int)floorf(fmaxf(i0, i1) + pre_in + 0.01f));
-        for (int j = j_lo; j <= j_hi; ++j) {
-            const int xi = xmajor ? o : j, yi = xmajor ? j : o;
-            // per-voxel arithmetic identical to the backprojection's
-            const float x = (float)xi - a.xpre, y = (float)yi - a.ypre;
-            if (!(x * x + y * y < a.r2)) continue;
-            float t = (-1.0f) * x * g.x + y * g.y;
-            t += a.uoff;
-            const float s = x * g.y + y * g.x;
-            const float U = cone_magnification(a.fcd, s);
-            const float u = t * U;
-            if (!(u >= a.ulo && u <= a.uhi)) continue;
-            const float uIdx = u + a.upre;
-            const int u0 = (int)uIdx;
-            if (u0 != c && u0 != c - 1) continue;
-            const float fu = uIdx - (float)u0;
-            const float wc = U * U * (u0 == c ? 1.0f - fu : fu);
-            // z whose row index lies in [r0 - 1, r0 + 2): vIndex ~ vPre - (z - zPre + h) U
-            const float iU = __builtin_amdgcn_rcpf(U);
-            const float za_f = (a.vpre - rf - 2.0f) * iU - h + a.zpre;
-            const float zb_f = (a.vpre - rf + 1.0f) * iU - h + a.zpre;
-            const int zlo = max(a.z0, (int)ceilf(za_f - 0.01f));
-            const int zhi = min(zend, (int)floorf(zb_f + 0.01f));
-            const float* colv = vol + ((size_t)xi * a.ny + yi) * a.nzs - a.z0;
-            int za_cur = -1;
-            float vA = 0.0f;
-            for (int zi = zlo; zi <= zhi; ++zi) {
-                const float zk = (float)zi - a.zpre;
-                const float v = (zk + h) * U;
-                if (!(v >= a.vlo && v <= a.vhi)) continue;
-                const int za = zi & ~(kRowAnchor - 1);
-                if (za != za_cur) {
-                    za_cur = za;
-                    vA = det_row_anchor((float)za - a.zpre, h, U, a.vpre);
+        jl = max(0, (int)ceilf(fminf(i0, i1) + pre_in - 0.01f));
+        jh = min(n_in - 1, (int)floorf(fmaxf(i0, i1) + pre_in + 0.01f));
+    };
+    // slots per depth step: the wedge width |i1 - i0| is linear in the depth and the
+    // source lies outside the volume, so it peaks at an end of the depth range; the
+    // count of integers in a window of width w + 0.02 is at most floor(w) + 2
+    auto width = [&](int o) {
+        const float ow = (float)o - pre_out;
+        return fabsf((ow - So) * (k1 - k0));
+    };
+    const int jm = (int)fmaxf(width(o_lo), width(o_hi)) + 2;
+    const int jm_pass = min(jm, kSlots);
+    const int per_pass = max(1, kSlots / jm_pass);
+
+    for (int ob = o_lo; ob <= o_hi; ob += per_pass) {
+        const int no = min(per_pass, o_hi - ob + 1);
+        for (int jb = 0; jb < jm; jb += jm_pass) {
+            const int nj = min(jm_pass, jm - jb);
+            const int nslot = no * nj;
+            // fill: slot (o - ob) * nj + jj covers lateral candidate jl + jb + jj of depth o;
+            // the candidates that pass the tests are stored compacted, in slot order
+            // (a ballot prefix within each wave, wave totals through LDS)
+            int ncand = 0;
+            for (int sbase = 0; sbase < nslot; sbase += kGThreads) {
+                const int sidx = sbase + tid;
+                const int oi = sidx / nj, jj = sidx - oi * nj;
+                const int o = ob + oi;
+                int jl = 0, jh = -1;
+                if (sidx < nslot) jrange(o, jl, jh);
+                const int j = jl + jb + jj;
+                int4 rec = make_int4(-1, 0, 0, 0);
+                if (sidx < nslot && j <= jh) {
+                    const int xi = xmajor ? o : j, yi = xmajor ? j : o;
+                    // per-voxel arithmetic identical to the backprojection's
+                    const float x = (float)xi - a.xpre, y = (float)yi - a.ypre;
+                    if (x * x + y * y < a.r2) {
+                        float t = (-1.0f) * x * g.x + y * g.y;
+                        t += a.uoff;
+                        const float s = x * g.y + y * g.x;
+                        const float U = cone_magnification(a.fcd, s);
+                        const float u = t * U;
+                        if (u >= a.ulo && u <= a.uhi) {
+                            const float uIdx = u + a.upre;
+                            const int u0 = (int)uIdx;
+                            if (u0 == c || u0 == c - 1) {
+                                const float fu = uIdx - (float)u0;
+                                const float wc = U * U * (u0 == c ? 1.0f - fu : fu);
+                                rec = make_int4(xi * a.ny + yi, __float_as_int(U), __float_as_int(wc), 0);
+                            }
+                        }
+                    }
                 }
-                const float vI = det_row_index((float)(zi - za), U, vA);
-                const int v0 = (int)vI;
-                const float fv = vI - (float)v0;
-                const float val = wc * colv[zi];
-                const int d = v0 - r0;
-                if (d == 0) {
-                    acc0 = fmaf(1.0f - fv, val, acc0);
-                    acc1 = fmaf(fv, val, acc1);
-                } else if (d == -1) {
-                    acc0 = fmaf(fv, val, acc0);
-                } else if (d == 1) {
-                    acc1 = fmaf(1.0f - fv, val, acc1);
+                const unsigned long long m = __ballot(rec.x >= 0);
+                const int below = __popcll(m & ((1ull << lane) - 1ull));
+                if (lane == 0) wave_cnt[wv] = __popcll(m);
+                __syncthreads();
+                int base = ncand;
+                for (int w = 0; w < wv; ++w) base += wave_cnt[w];
+                int total = 0;
+                for (int w = 0; w < kGWaves; ++w) total += wave_cnt[w];
+                if (rec.x >= 0) slots[base + below] = rec;
+                ncand += total;
+                __syncthreads();  // wave_cnt is rewritten by the next round
+            }
+            // walk: wave wv takes the 64-voxel chunks wv, wv + 4, ... of every candidate, two
+            // at a time (their rows are far apart); a voxel's taps go to rows v0, v0 + 1
+            for (int ci = 0; ci < ncand; ++ci) {
+                const int4 rec = slots[ci];
+                const float U = __int_as_float(rec.y), wc = __int_as_float(rec.z);
+                const float* colv = vol + (size_t)rec.x * a.nzs - a.z0;
+                // lanes k apart have rows at least k*U - 1 apart: k = ceil(2/U) residues
+                const int npass = (int)ceilf(2.0f * __builtin_amdgcn_rcpf(U) + 1e-3f);
+                auto tap = [&](int zi, float& ta, float& tb) -> int {
+                    ta = 0.0f;
+                    tb = 0.0f;
+                    int v0 = H;  // off the image: rows H, H + 1 take zero
+                    if (zi <= zend) {
+                        const float zk = (float)zi - a.zpre;
+                        const float v = (zk + h) * U;
+                        if (v >= a.vlo && v <= a.vhi) {
+                            const int za = zi & ~(kRowAnchor - 1);
+                            const float vA = det_row_anchor((float)za - a.zpre, h, U, a.vpre);
+                            const float vI = det_row_index((float)(zi - za), U, vA);
+                            v0 = (int)vI;
+                            const float fv = vI - (float)v0;
+                            const float val = wc * colv[zi];
+                            ta = (1.0f - fv) * val;
+                            tb = fv * val;
+                            v0 = min(max(v0, -2), H);
+                        }
+                    }
+                    return v0 + 2;
+                };
+                for (int zc = a.z0 + 64 * wv; zc <= zend; zc += 128 * kGWaves) {
+                    float ta0, tb0, ta1, tb1;
+                    const int i0 = tap(zc + lane, ta0, tb0);
+                    const int i1 = tap(zc + 64 * kGWaves + lane, ta1, tb1);
+                    for (int ps = 0; ps < npass; ++ps) {
+                        if (lane % npass == ps) {
+                            // loads before stores: the four rows are distinct
+                            const float a0 = myrows[i0], a1 = myrows[i0 + 1];
+                            const float b0 = myrows[i1], b1 = myrows[i1 + 1];
+                            myrows[i0] = a0 + ta0;
+                            myrows[i0 + 1] = a1 + tb0;
+                            myrows[i1] = b0 + ta1;
+                            myrows[i1 + 1] = b1 + tb1;
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                    }
                 }
             }
+            __syncthreads();
         }
     }
-    float* px = proj + ((size_t)p * a.H + r0) * a.W + c;
-    if (r0 < a.H) px[0] += acc0;
-    if (r0 + 1 < a.H) px[a.W] += acc1;
+    // rows of the four waves, summed in wave order, into the image
+    for (int r = tid; r < H; r += kGThreads) {
+        float sum = rows_lds[r + 2];
+        for (int w = 1; w < kGWaves; ++w) sum += rows_lds[(size_t)w * pitch + r + 2];
+        proj[((size_t)p * H + r) * a.W + c] += sum;
+    }
 }
 
 // ------------------------------------------------------------------ phantom
@@ -259,9 +338,11 @@ ctg_status launch_forward_project(const ctg_geometry& g, const float4* d_params,
     a.xpre = g.x_pre; a.ypre = g.y_pre; a.zpre = g.z_pre; a.upre = g.u_pre; a.vpre = g.v_pre;
     a.r2 = g.radius * g.radius; a.fcd = g.fcd; a.uoff = g.u_offset;
     a.ulo = g.u_lo; a.uhi = g.u_hi; a.vlo = g.v_lo; a.vhi = g.v_hi;
-    dim3 grid((unsigned)a.W, (unsigned)((a.H + kGRows - 1) / kGRows), (unsigned)nproj);
-    if (grid.y > 65535u || grid.z > 65535u) return fail(CTG_ERROR_UNSUPPORTED, "forward projection grid too large");
-    hipLaunchKernelGGL(fp_gather_kernel, grid, dim3(kGThreads), 0, st, a, d_params, d_volume, d_proj);
+    if (a.H > kMaxRowsFp) return fail(CTG_ERROR_UNSUPPORTED, "forward projection of more than %d detector rows", kMaxRowsFp);
+    dim3 grid((unsigned)a.W, (unsigned)nproj);
+    if (grid.y > 65535u) return fail(CTG_ERROR_UNSUPPORTED, "forward projection grid too large");
+    const size_t lds = sizeof(float) * (size_t)kGWaves * (a.H + 4);
+    hipLaunchKernelGGL(fp_gather_kernel, grid, dim3(kGThreads), lds, st, a, d_params, d_volume, d_proj);
     CTG_HIP(hipGetLastError());
     return CTG_OK;
 }
