@@ -33,7 +33,9 @@ struct FpArgs {
 //    candidate: the lanes fill a compacted list of candidates in LDS, in walk order.
 //  - Each wave then takes 64 consecutive z of a candidate, one per lane, with the
 //    backprojection's own arithmetic (masks, anchored row index), and adds the two row
-//    taps of its voxel into the wave's own row accumulators in LDS.  Rows of lanes k
+//    taps of its voxel into the wave's own row accumulators in LDS.  Slabs of 512 z or
+//    more give each wave two far-apart chunks of every candidate at once; thinner slabs
+//    hand (candidate, chunk) items to the waves round-robin.  Rows of lanes k
 //    apart differ by at least k*U - 1, so lanes are split into k = ceil(2/U) residue
 //    passes whose taps never share a row; the read-modify-writes need no atomics.
 //  - At the end the four waves' rows are summed into the image.
@@ -145,39 +147,62 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
                 ncand += total;
                 __syncthreads();  // wave_cnt is rewritten by the next round
             }
-            // walk: wave wv takes the 64-voxel chunks wv, wv + 4, ... of every candidate, two
-            // at a time (their rows are far apart); a voxel's taps go to rows v0, v0 + 1
-            for (int ci = 0; ci < ncand; ++ci) {
-                const int4 rec = slots[ci];
-                const float U = __int_as_float(rec.y), wc = __int_as_float(rec.z);
-                const float* colv = vol + (size_t)rec.x * a.nzs - a.z0;
-                // lanes k apart have rows at least k*U - 1 apart: k = ceil(2/U) residues
-                const int npass = (int)ceilf(2.0f * __builtin_amdgcn_rcpf(U) + 1e-3f);
-                auto tap = [&](int zi, float& ta, float& tb) -> int {
-                    ta = 0.0f;
-                    tb = 0.0f;
-                    int v0 = H;  // off the image: rows H, H + 1 take zero
-                    if (zi <= zend) {
-                        const float zk = (float)zi - a.zpre;
-                        const float v = (zk + h) * U;
-                        if (v >= a.vlo && v <= a.vhi) {
-                            const int za = zi & ~(kRowAnchor - 1);
-                            const float vA = det_row_anchor((float)za - a.zpre, h, U, a.vpre);
-                            const float vI = det_row_index((float)(zi - za), U, vA);
-                            v0 = (int)vI;
-                            const float fv = vI - (float)v0;
-                            const float val = wc * colv[zi];
-                            ta = (1.0f - fv) * val;
-                            tb = fv * val;
-                            v0 = min(max(v0, -2), H);
-                        }
+            // walk: a voxel's taps go to rows v0, v0 + 1 of the wave's own accumulators
+            const float* volz = vol - a.z0;
+            auto tap = [&](int zi, float xv, float U, float wc, float& ta, float& tb) -> int {
+                ta = 0.0f;
+                tb = 0.0f;
+                int v0 = H;  // off the image: rows H, H + 1 take zero
+                if (zi <= zend) {
+                    const float zk = (float)zi - a.zpre;
+                    const float v = (zk + h) * U;
+                    if (v >= a.vlo && v <= a.vhi) {
+                        const int za = zi & ~(kRowAnchor - 1);
+                        const float vA = det_row_anchor((float)za - a.zpre, h, U, a.vpre);
+                        const float vI = det_row_index((float)(zi - za), U, vA);
+                        v0 = (int)vI;
+                        const float fv = vI - (float)v0;
+                        const float val = wc * xv;
+                        ta = (1.0f - fv) * val;
+                        tb = fv * val;
+                        v0 = min(max(v0, -2), H);
                     }
-                    return v0 + 2;
+                }
+                return v0 + 2;
+            };
+            // lanes k apart have rows at least k*U - 1 apart: k = ceil(2/U) residues
+            auto passes = [&](float U) { return (int)ceilf(2.0f * __builtin_amdgcn_rcpf(U) + 1e-3f); };
+            if (a.nzs >= 128 * kGWaves) {
+                // Wave wv takes the 64-voxel chunks wv, wv + 4, ... of every candidate, two
+                // at a time (their rows are far apart).  Items are (candidate, chunk pair)
+                // in walk order; the voxels of the next item are loaded while the current
+                // one is processed.
+                const int span = 128 * kGWaves;  // z covered by one chunk pair of every wave
+                const int n_it = a.nzs > 64 * wv ? (a.nzs - 64 * wv + span - 1) / span : 0;
+                const int nitems = ncand * n_it;
+                auto item_load = [&](int q, int4& r, int& zc, float& x0, float& x1) {
+                    const int ci = q / n_it, it = q - ci * n_it;
+                    r = slots[ci];
+                    zc = a.z0 + 64 * wv + it * span;
+                    const float* colv = volz + (size_t)r.x * a.nzs;
+                    const int za = zc + lane, zb = zc + 64 * kGWaves + lane;
+                    x0 = za <= zend ? colv[za] : 0.0f;
+                    x1 = zb <= zend ? colv[zb] : 0.0f;
                 };
-                for (int zc = a.z0 + 64 * wv; zc <= zend; zc += 128 * kGWaves) {
+                int4 rec_n = make_int4(0, 0, 0, 0);
+                int zc_n = 0;
+                float x0_n = 0.0f, x1_n = 0.0f;
+                if (nitems > 0) item_load(0, rec_n, zc_n, x0_n, x1_n);
+                for (int q = 0; q < nitems; ++q) {
+                    const int4 rec = rec_n;
+                    const int zc = zc_n;
+                    const float x0 = x0_n, x1 = x1_n;
+                    if (q + 1 < nitems) item_load(q + 1, rec_n, zc_n, x0_n, x1_n);
+                    const float U = __int_as_float(rec.y), wc = __int_as_float(rec.z);
+                    const int npass = passes(U);
                     float ta0, tb0, ta1, tb1;
-                    const int i0 = tap(zc + lane, ta0, tb0);
-                    const int i1 = tap(zc + 64 * kGWaves + lane, ta1, tb1);
+                    const int i0 = tap(zc + lane, x0, U, wc, ta0, tb0);
+                    const int i1 = tap(zc + 64 * kGWaves + lane, x1, U, wc, ta1, tb1);
                     for (int ps = 0; ps < npass; ++ps) {
                         if (lane % npass == ps) {
                             // loads before stores: the four rows are distinct
@@ -187,6 +212,40 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
                             myrows[i0 + 1] = a1 + tb0;
                             myrows[i1] = b0 + ta1;
                             myrows[i1 + 1] = b1 + tb1;
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                }
+            } else {
+                // Thin slabs: items (candidate, 64-voxel chunk) in walk order go to the waves
+                // round-robin, one chunk at a time, so that every wave has work.
+                const int nch = (a.nzs + 63) / 64;
+                const int nitems = ncand * nch;
+                auto item_load = [&](int q, int4& r, int& zc, float& x0) {
+                    const int ci = q / nch, k = q - ci * nch;
+                    r = slots[ci];
+                    zc = a.z0 + 64 * k;
+                    const int za = zc + lane;
+                    x0 = za <= zend ? volz[(size_t)r.x * a.nzs + za] : 0.0f;
+                };
+                int4 rec_n = make_int4(0, 0, 0, 0);
+                int zc_n = 0;
+                float x0_n = 0.0f;
+                if (wv < nitems) item_load(wv, rec_n, zc_n, x0_n);
+                for (int q = wv; q < nitems; q += kGWaves) {
+                    const int4 rec = rec_n;
+                    const int zc = zc_n;
+                    const float x0 = x0_n;
+                    if (q + kGWaves < nitems) item_load(q + kGWaves, rec_n, zc_n, x0_n);
+                    const float U = __int_as_float(rec.y), wc = __int_as_float(rec.z);
+                    const int npass = passes(U);
+                    float ta0, tb0;
+                    const int i0 = tap(zc + lane, x0, U, wc, ta0, tb0);
+                    for (int ps = 0; ps < npass; ++ps) {
+                        if (lane % npass == ps) {
+                            const float a0 = myrows[i0], a1 = myrows[i0 + 1];
+                            myrows[i0] = a0 + ta0;
+                            myrows[i0 + 1] = a1 + tb0;
                         }
                         __builtin_amdgcn_wave_barrier();
                     }
