@@ -288,6 +288,41 @@ def test_forward_projector_matches_fp64_oracle(gpu, oracle):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("nz", [40, 520])
+def test_forward_projector_wide_cone_and_slab_widths(gpu, oracle, nz):
+    """The widest cone the geometry allows (FCD just above the detector width) makes the
+    candidate wedge of a detector column widest; a 40-slice volume takes the projector's
+    thin-slab walk and a 520-slice one its paired-chunk walk (partial last chunks)."""
+    import torch
+    import ct_amd
+    W, H, P = 64, 48, 10
+    g = ct_amd.geometry_grid(W, H, 65.0, 0.0, 40, 40, nz)
+    ctx = ct_amd.Context(0)
+    ctx.set_geometry(g)
+    angles = np.linspace(0.0, 360.0, P, endpoint=False)
+    heights = np.linspace(-2.0, 2.0, P)
+    ctx.set_projections(angles, heights)
+    rng = np.random.default_rng(nz)
+    x = rng.standard_normal((g.nx, g.ny, g.nz)).astype(np.float32)
+    y = rng.standard_normal((P, H, W)).astype(np.float32)
+    xd, yd = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    Ax = torch.empty((P, H, W), dtype=torch.float32, device="cuda")
+    ctx.forward_project(xd, 0, g.nz, 0, P, Ax)
+    yl = torch.empty(ctx.filtered_bytes("f32", P), dtype=torch.uint8, device="cuda")
+    ctx.layout_projections(yd, P, "f32", yl, None)
+    Aty = torch.empty_like(xd)
+    ctx.backproject(yl, "f32", None, 0, P, Aty, 0, g.nz)
+    torch.cuda.synchronize()
+    lhs = float(np.sum(Ax.cpu().numpy().astype(np.float64) * y))
+    rhs = float(np.sum(x.astype(np.float64) * Aty.cpu().numpy()))
+    rel = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    ref = oracle.forward_project_f64(_oracle_geom(g), x.astype(np.float64), angles, heights)
+    err = rel_rmse(Ax.cpu().numpy(), ref)
+    print(f"wide cone nz={nz}: dot {rel:.2e}, vs FP64 oracle {err:.2e}")
+    assert rel < 1e-5
+    assert err < 1e-5
+
+
 def test_phantom_generator_matches_oracle(gpu, oracle):
     import torch
     scan = small_scan(W=64, H=48, P=6, seed=12)
