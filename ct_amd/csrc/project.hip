@@ -17,6 +17,7 @@ namespace {
 
 struct FpArgs {
     int W, H, nx, ny, nzs, z0, nproj;
+    int zstride;  // slices per voxel column of the volume buffer (nzs of them from z0 are projected)
     float xpre, ypre, zpre, upre, vpre;
     float r2, fcd, uoff, ulo, uhi, vlo, vhi;
 };
@@ -31,19 +32,43 @@ struct FpArgs {
 //    the candidates up to rounding; a 0.01-voxel margin covers that.
 //  - The per-column work (cylinder and window tests, U, the tap weight) is done once per
 //    candidate: the lanes fill a compacted list of candidates in LDS, in walk order.
+//    The v-window test (v = (z + h) U inside [vLo, vHi]) is monotonic in z for U > 0, so
+//    the fill also turns it, intersected with the slab, into an exact interval of z
+//    (found by evaluating the backprojection's own expression at the boundaries);
+//    candidates whose interval is empty are dropped.
 //  - Each wave then takes 64 consecutive z of a candidate, one per lane, with the
-//    backprojection's own arithmetic (masks, anchored row index), and adds the two row
-//    taps of its voxel into the wave's own row accumulators in LDS.  Slabs of 512 z or
-//    more give each wave two far-apart chunks of every candidate at once; thinner slabs
-//    hand (candidate, chunk) items to the waves round-robin.  Rows of lanes k
-//    apart differ by at least k*U - 1, so lanes are split into k = ceil(2/U) residue
-//    passes whose taps never share a row; the read-modify-writes need no atomics.
+//    backprojection's anchored row index, and adds the two row taps of its voxel into the
+//    wave's own row accumulators in LDS.  Chunks start at multiples of 32 in grid z, so a
+//    lane's row anchor and its offset from it are fixed per lane.  Slabs of 512 z or more
+//    give each wave two far-apart chunks of every candidate at once; thinner slabs hand
+//    (candidate, chunk) items to the waves round-robin.  Rows of lanes k apart differ by
+//    at least k*U - 1, so lanes are split into k = ceil(2/U) residue passes whose taps
+//    never share a row; the read-modify-writes need no atomics.
 //  - At the end the four waves' rows are summed into the image.
 // Every (voxel, tap) pair is added exactly once, and the order of all additions is fixed:
 // the result does not depend on scheduling.
 constexpr int kGThreads = 256, kGWaves = kGThreads / 64;
 constexpr int kSlots = 1024;  // candidate slots per pass (16 B each)
 constexpr int kMaxRowsFp = 4096;  // detector rows held in LDS per wave
+constexpr int kFpZBits = 14;      // z interval bounds in a candidate record (relative to the run base)
+constexpr int kFpMaxSlab = (1 << kFpZBits) - 2 * kRowAnchor;  // slices per launch
+constexpr int kFpMaxPasses = 8;   // residue passes held in the per-lane table
+
+// Raw buffer resource over `bytes` bytes of global memory: a 4-byte load at an offset at
+// or past the end returns zero.  The builtins exist only in the device pass.
+#if defined(__HIP_DEVICE_COMPILE__)
+using vbuf = __amdgpu_buffer_rsrc_t;
+__device__ inline vbuf make_vbuf(const float* base, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), 0, bytes, 0x00020000);
+}
+__device__ inline float vbuf_load(vbuf rs, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, 0));
+}
+#else
+struct vbuf {};
+__device__ inline vbuf make_vbuf(const float*, int) { return {}; }
+__device__ inline float vbuf_load(vbuf, unsigned) { return 0.0f; }
+#endif
 
 __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const float4* __restrict__ pp,
                                                               const float* __restrict__ vol,
@@ -58,6 +83,10 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
     const int H = a.H, pitch = a.H + 4;
     float* myrows = rows_lds + (size_t)wv * pitch;
     for (int i = tid; i < kGWaves * pitch; i += kGThreads) rows_lds[i] = 0.0f;
+    // lane % n for n = 1..8, four bits each: this lane's residue pass for n passes
+    unsigned pass_bits = 0;
+#pragma unroll
+    for (int n = 1; n <= kFpMaxPasses; ++n) pass_bits |= (unsigned)(lane % n) << (4 * (n - 1));
     __syncthreads();
 
     const float4 g = pp[p];
@@ -78,6 +107,7 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
     const float R = sqrtf(a.r2);
     const int o_lo = max(0, (int)floorf(pre_out - R) - 1), o_hi = min(n_out - 1, (int)ceilf(pre_out + R) + 1);
     const int zend = a.z0 + a.nzs - 1;  // last grid z index of the slab
+    const int zrb = a.z0 & ~(kRowAnchor - 1);  // run base: chunks start at zrb + 64 k
 
     // lateral candidates of depth step o: [jl, jh]
     auto jrange = [&](int o, int& jl, int& jh) {
@@ -130,7 +160,31 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
                             if (u0 == c || u0 == c - 1) {
                                 const float fu = uIdx - (float)u0;
                                 const float wc = U * U * (u0 == c ? 1.0f - fu : fu);
-                                rec = make_int4(xi * a.ny + yi, __float_as_int(U), __float_as_int(wc), 0);
+                                // z interval of the window test v = (zk + h) U in [vLo, vHi]:
+                                // v is non-decreasing in z (U > 0, monotone rounding), so the
+                                // passing z form an interval; estimate its ends, then settle
+                                // them with the backprojection's own expression
+                                auto v_of = [&](int zi) { const float zk = (float)zi - a.zpre; return (zk + h) * U; };
+                                const float zlo_f = fminf(fmaxf(a.vlo / U - h + a.zpre, (float)(a.z0 - 4)), (float)(zend + 4));
+                                const float zhi_f = fminf(fmaxf(a.vhi / U - h + a.zpre, (float)(a.z0 - 4)), (float)(zend + 4));
+                                int lo = (int)ceilf(zlo_f), hi = (int)floorf(zhi_f);
+#pragma unroll
+                                for (int it = 0; it < 4; ++it) {
+                                    if (lo > a.z0 && v_of(lo - 1) >= a.vlo) --lo;
+                                    if (hi < zend && v_of(hi + 1) <= a.vhi) ++hi;
+                                }
+#pragma unroll
+                                for (int it = 0; it < 4; ++it) {
+                                    if (lo <= zend && !(v_of(lo) >= a.vlo)) ++lo;
+                                    if (hi >= a.z0 && !(v_of(hi) <= a.vhi)) --hi;
+                                }
+                                lo = max(lo, a.z0);
+                                hi = min(hi, zend);
+                                if (lo <= hi) {
+                                    const int npass = (int)ceilf(2.0f * __builtin_amdgcn_rcpf(U) + 1e-3f);
+                                    rec = make_int4(xi * a.ny + yi, __float_as_int(U), __float_as_int(wc),
+                                                    (lo - zrb) | ((hi - zrb) << kFpZBits) | (npass << (2 * kFpZBits)));
+                                }
                             }
                         }
                     }
@@ -147,64 +201,87 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
                 ncand += total;
                 __syncthreads();  // wave_cnt is rewritten by the next round
             }
-            // walk: a voxel's taps go to rows v0, v0 + 1 of the wave's own accumulators
-            const float* volz = vol - a.z0;
-            auto tap = [&](int zi, float xv, float U, float wc, float& ta, float& tb) -> int {
-                ta = 0.0f;
-                tb = 0.0f;
-                int v0 = H;  // off the image: rows H, H + 1 take zero
-                if (zi <= zend) {
-                    const float zk = (float)zi - a.zpre;
-                    const float v = (zk + h) * U;
-                    if (v >= a.vlo && v <= a.vhi) {
-                        const int za = zi & ~(kRowAnchor - 1);
-                        const float vA = det_row_anchor((float)za - a.zpre, h, U, a.vpre);
-                        const float vI = det_row_index((float)(zi - za), U, vA);
-                        v0 = (int)vI;
-                        const float fv = vI - (float)v0;
-                        const float val = wc * xv;
-                        ta = (1.0f - fv) * val;
-                        tb = fv * val;
-                        v0 = min(max(v0, -2), H);
-                    }
-                }
-                return v0 + 2;
+            // walk: a voxel's taps go to rows v0, v0 + 1 of the wave's own accumulators.
+            // Chunks start at zrb + 64 k (multiples of 32 in grid z), so lane l's voxel
+            // lies (l & 31) slices past its row anchor, which is (l & 32) slices past the
+            // chunk start.  Lanes outside the candidate's z interval load zero and send
+            // their (zero) taps to the absorbing rows H, H + 1: their own rows could meet
+            // an active lane's, because truncation folds (-1, 1) onto row 0.  A lane whose
+            // voxel is zero may do the same, as its taps are zero too.
+            const float lane_k = (float)(lane & 31), anchor_off = (float)(lane & 32);
+            const float* colbase = vol + (zrb - a.z0);  // voxel (column, z) at column * zstride + z - zrb
+            auto tap = [&](float zcw, float xv, float U, float wc, float& ta, float& tb) -> int {
+                const float vA = det_row_anchor(zcw + anchor_off, h, U, a.vpre);
+                const float vI = det_row_index(lane_k, U, vA);
+                const int v0 = (int)vI;
+                const float fv = vI - (float)v0;
+                const float val = wc * xv;
+                tb = fv * val;
+                ta = val - tb;
+                return (xv != 0.0f ? min(max(v0, -2), H) : H) + 2;
             };
-            // lanes k apart have rows at least k*U - 1 apart: k = ceil(2/U) residues
-            auto passes = [&](float U) { return (int)ceilf(2.0f * __builtin_amdgcn_rcpf(U) + 1e-3f); };
-            if (a.nzs >= 128 * kGWaves) {
+            // a candidate record: column, U, weight, and z interval [lo, hi] relative to
+            // zrb with the residue pass count in the top bits.  The column's voxels in
+            // [lo, hi] are read through a buffer resource covering exactly them: a load
+            // outside it returns zero, which is the mask of the window and the slab.
+            struct Item {
+                vbuf rs;
+                float U, wc;
+                int lo, npass;
+            };
+            // past the last candidate an item still issues its loads, over an empty
+            // resource (they return zero without touching memory): every slot then has
+            // the same loads in flight and the walk waits for exactly its own
+            auto item_of = [&](int ci, bool valid) {
+                const int4 r = slots[valid ? ci : 0];
+                Item it;
+                const int w = __builtin_amdgcn_readfirstlane(r.w);
+                it.lo = w & ((1 << kFpZBits) - 1);
+                const int hi = (w >> kFpZBits) & ((1 << kFpZBits) - 1);
+                it.rs = make_vbuf(colbase + (size_t)__builtin_amdgcn_readfirstlane(r.x) * a.zstride + it.lo,
+                                  valid ? (hi - it.lo + 1) * (int)sizeof(float) : 0);
+                it.U = __int_as_float(__builtin_amdgcn_readfirstlane(r.y));
+                it.wc = __int_as_float(__builtin_amdgcn_readfirstlane(r.z));
+                it.npass = (unsigned)w >> (2 * kFpZBits);
+                return it;
+            };
+            // this lane's voxel of the chunk starting at zc (relative to zrb)
+            auto load = [&](const Item& it, int zc) -> float {
+                return vbuf_load(it.rs, (unsigned)(zc - it.lo + lane) * (unsigned)sizeof(float));
+            };
+            auto zworld = [&](int zc) { return (float)(zrb + zc) - a.zpre; };
+            const int nzr = zend - zrb + 1;  // slices from the run base to the slab end
+            if (ncand == 0) {
+                __syncthreads();
+                continue;
+            }
+            // Each wave walks its items in order through three slots: the voxels of the
+            // next two items are in flight while the current one is processed (the slots
+            // are named, not rotated, so that no in-flight register is copied).
+            if (nzr >= 128 * kGWaves) {
                 // Wave wv takes the 64-voxel chunks wv, wv + 4, ... of every candidate, two
-                // at a time (their rows are far apart).  Items are (candidate, chunk pair)
-                // in walk order; the voxels of the next item are loaded while the current
-                // one is processed.
+                // at a time (their rows are far apart): items are (candidate, chunk pair).
                 const int span = 128 * kGWaves;  // z covered by one chunk pair of every wave
-                const int n_it = a.nzs > 64 * wv ? (a.nzs - 64 * wv + span - 1) / span : 0;
-                const int nitems = ncand * n_it;
-                auto item_load = [&](int q, int4& r, int& zc, float& x0, float& x1) {
-                    const int ci = q / n_it, it = q - ci * n_it;
-                    r = slots[ci];
-                    zc = a.z0 + 64 * wv + it * span;
-                    const float* colv = volz + (size_t)r.x * a.nzs;
-                    const int za = zc + lane, zb = zc + 64 * kGWaves + lane;
-                    x0 = za <= zend ? colv[za] : 0.0f;
-                    x1 = zb <= zend ? colv[zb] : 0.0f;
+                const int n_it = nzr > 64 * wv ? (nzr - 64 * wv + span - 1) / span : 0;
+                int ci_f = 0, k_f = 0;  // next item to fetch
+                struct Slot { Item it; int zc; float x0, x1; };
+                auto next = [&](Slot& s) -> bool {
+                    const bool valid = ci_f < ncand;
+                    s.it = item_of(ci_f, valid);
+                    s.zc = 64 * wv + k_f * span;
+                    s.x0 = load(s.it, s.zc);
+                    s.x1 = load(s.it, s.zc + 64 * kGWaves);
+                    if (++k_f == n_it) { k_f = 0; ++ci_f; }
+                    __builtin_amdgcn_sched_barrier(0);  // keep the slots' loads in walk order
+                    return valid;
                 };
-                int4 rec_n = make_int4(0, 0, 0, 0);
-                int zc_n = 0;
-                float x0_n = 0.0f, x1_n = 0.0f;
-                if (nitems > 0) item_load(0, rec_n, zc_n, x0_n, x1_n);
-                for (int q = 0; q < nitems; ++q) {
-                    const int4 rec = rec_n;
-                    const int zc = zc_n;
-                    const float x0 = x0_n, x1 = x1_n;
-                    if (q + 1 < nitems) item_load(q + 1, rec_n, zc_n, x0_n, x1_n);
-                    const float U = __int_as_float(rec.y), wc = __int_as_float(rec.z);
-                    const int npass = passes(U);
+                auto process = [&](const Slot& s) {
                     float ta0, tb0, ta1, tb1;
-                    const int i0 = tap(zc + lane, x0, U, wc, ta0, tb0);
-                    const int i1 = tap(zc + 64 * kGWaves + lane, x1, U, wc, ta1, tb1);
-                    for (int ps = 0; ps < npass; ++ps) {
-                        if (lane % npass == ps) {
+                    const int i0 = tap(zworld(s.zc), s.x0, s.it.U, s.it.wc, ta0, tb0);
+                    const int i1 = tap(zworld(s.zc + 64 * kGWaves), s.x1, s.it.U, s.it.wc, ta1, tb1);
+                    const int pid = (int)((pass_bits >> (4 * (s.it.npass - 1))) & 15u);
+                    for (int ps = 0; ps < s.it.npass; ++ps) {
+                        if (pid == ps) {
                             // loads before stores: the four rows are distinct
                             const float a0 = myrows[i0], a1 = myrows[i0 + 1];
                             const float b0 = myrows[i1], b1 = myrows[i1 + 1];
@@ -215,40 +292,60 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
                         }
                         __builtin_amdgcn_wave_barrier();
                     }
+                };
+                Slot sa, sb;
+                bool ha = next(sa), hb = next(sb);
+                while (ha) {
+                    process(sa);
+                    ha = next(sa);
+                    if (!hb) break;
+                    process(sb);
+                    hb = next(sb);
                 }
             } else {
                 // Thin slabs: items (candidate, 64-voxel chunk) in walk order go to the waves
-                // round-robin, one chunk at a time, so that every wave has work.
-                const int nch = (a.nzs + 63) / 64;
-                const int nitems = ncand * nch;
-                auto item_load = [&](int q, int4& r, int& zc, float& x0) {
-                    const int ci = q / nch, k = q - ci * nch;
-                    r = slots[ci];
-                    zc = a.z0 + 64 * k;
-                    const int za = zc + lane;
-                    x0 = za <= zend ? volz[(size_t)r.x * a.nzs + za] : 0.0f;
+                // round-robin, one chunk at a time, so that every wave has work.  Item
+                // q = ci * nch + k; wave wv takes q = wv, wv + 4, ...
+                const int nch = (nzr + 63) / 64;
+                const int dci = kGWaves / nch, dk = kGWaves - dci * nch;
+                int ci_f = 0, k_f = wv;
+                while (k_f >= nch) { k_f -= nch; ++ci_f; }
+                struct Slot { Item it; int zc; float x0; };
+                auto next = [&](Slot& s) -> bool {
+                    const bool valid = ci_f < ncand;
+                    s.it = item_of(ci_f, valid);
+                    s.zc = 64 * k_f;
+                    s.x0 = load(s.it, s.zc);
+                    ci_f += dci;
+                    k_f += dk;
+                    if (k_f >= nch) { k_f -= nch; ++ci_f; }
+                    __builtin_amdgcn_sched_barrier(0);  // keep the slots' loads in walk order
+                    return valid;
                 };
-                int4 rec_n = make_int4(0, 0, 0, 0);
-                int zc_n = 0;
-                float x0_n = 0.0f;
-                if (wv < nitems) item_load(wv, rec_n, zc_n, x0_n);
-                for (int q = wv; q < nitems; q += kGWaves) {
-                    const int4 rec = rec_n;
-                    const int zc = zc_n;
-                    const float x0 = x0_n;
-                    if (q + kGWaves < nitems) item_load(q + kGWaves, rec_n, zc_n, x0_n);
-                    const float U = __int_as_float(rec.y), wc = __int_as_float(rec.z);
-                    const int npass = passes(U);
+                auto process = [&](const Slot& s) {
                     float ta0, tb0;
-                    const int i0 = tap(zc + lane, x0, U, wc, ta0, tb0);
-                    for (int ps = 0; ps < npass; ++ps) {
-                        if (lane % npass == ps) {
+                    const int i0 = tap(zworld(s.zc), s.x0, s.it.U, s.it.wc, ta0, tb0);
+                    const int pid = (int)((pass_bits >> (4 * (s.it.npass - 1))) & 15u);
+                    for (int ps = 0; ps < s.it.npass; ++ps) {
+                        if (pid == ps) {
                             const float a0 = myrows[i0], a1 = myrows[i0 + 1];
                             myrows[i0] = a0 + ta0;
                             myrows[i0 + 1] = a1 + tb0;
                         }
                         __builtin_amdgcn_wave_barrier();
                     }
+                };
+                Slot sa, sb, sc;
+                bool ha = next(sa), hb = next(sb), hc = next(sc);
+                while (ha) {
+                    process(sa);
+                    ha = next(sa);
+                    if (!hb) break;
+                    process(sb);
+                    hb = next(sb);
+                    if (!hc) break;
+                    process(sc);
+                    hc = next(sc);
                 }
             }
             __syncthreads();
@@ -397,12 +494,21 @@ ctg_status launch_forward_project(const ctg_geometry& g, const float4* d_params,
     a.xpre = g.x_pre; a.ypre = g.y_pre; a.zpre = g.z_pre; a.upre = g.u_pre; a.vpre = g.v_pre;
     a.r2 = g.radius * g.radius; a.fcd = g.fcd; a.uoff = g.u_offset;
     a.ulo = g.u_lo; a.uhi = g.u_hi; a.vlo = g.v_lo; a.vhi = g.v_hi;
+    a.zstride = nz;
     if (a.H > kMaxRowsFp) return fail(CTG_ERROR_UNSUPPORTED, "forward projection of more than %d detector rows", kMaxRowsFp);
+    // U = FCD / (FCD - s) with |s| < radius stays in (1/1.9, 10): at most 4 residue passes
+    if (!(g.radius < 0.9f * g.fcd))
+        return fail(CTG_ERROR_UNSUPPORTED, "forward projection needs the reconstruction cylinder within 0.9 FCD");
     dim3 grid((unsigned)a.W, (unsigned)nproj);
     if (grid.y > 65535u) return fail(CTG_ERROR_UNSUPPORTED, "forward projection grid too large");
     const size_t lds = sizeof(float) * (size_t)kGWaves * (a.H + 4);
-    hipLaunchKernelGGL(fp_gather_kernel, grid, dim3(kGThreads), lds, st, a, d_params, d_volume, d_proj);
-    CTG_HIP(hipGetLastError());
+    // the candidate records hold z relative to the launch's run base in kFpZBits bits
+    for (int s = 0; s < nz; s += kFpMaxSlab) {
+        a.z0 = z0 + s;
+        a.nzs = std::min(kFpMaxSlab, nz - s);
+        hipLaunchKernelGGL(fp_gather_kernel, grid, dim3(kGThreads), lds, st, a, d_params, d_volume + s, d_proj);
+        CTG_HIP(hipGetLastError());
+    }
     return CTG_OK;
 }
 
