@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace ctg {
 namespace {
@@ -36,13 +37,13 @@ struct FpArgs {
 //    the fill also turns it, intersected with the slab, into an exact interval of z
 //    (found by evaluating the backprojection's own expression at the boundaries);
 //    candidates whose interval is empty are dropped.
-//  - Each wave then takes 64 consecutive z of a candidate, one per lane, with the
-//    backprojection's anchored row index, and adds the two row taps of its voxel into the
-//    wave's own row accumulators in LDS.  Chunks start at multiples of 32 in grid z, so a
-//    lane's row anchor and its offset from it are fixed per lane.  Slabs of 512 z or more
-//    give each wave two far-apart chunks of every candidate at once; thinner slabs hand
-//    (candidate, chunk) items to the waves round-robin.  Rows of lanes k apart differ by
-//    at least k*U - 1, so lanes are split into k = ceil(2/U) residue passes whose taps
+//  - The waves then walk items of up to four consecutive 64-voxel z chunks of a
+//    candidate (one voxel per lane and chunk), dealt round-robin, with the
+//    backprojection's anchored row index, and add the two row taps of each voxel into
+//    the wave's own row accumulators in LDS.  Chunks start at multiples of 32 in grid z,
+//    so a lane's row anchor and its offset from it are fixed per lane.  Rows of voxels k
+//    slices apart differ by at least k*U - 1, so the lanes are split by residue into
+//    ceil(2/U) passes, rounded up to 2 or 4 (which divide the chunk length), whose taps
 //    never share a row; the read-modify-writes need no atomics.
 //  - At the end the four waves' rows are summed into the image.
 // Every (voxel, tap) pair is added exactly once, and the order of all additions is fixed:
@@ -52,7 +53,6 @@ constexpr int kSlots = 1024;  // candidate slots per pass (16 B each)
 constexpr int kMaxRowsFp = 4096;  // detector rows held in LDS per wave
 constexpr int kFpZBits = 14;      // z interval bounds in a candidate record (relative to the run base)
 constexpr int kFpMaxSlab = (1 << kFpZBits) - 2 * kRowAnchor;  // slices per launch
-constexpr int kFpMaxPasses = 8;   // residue passes held in the per-lane table
 
 // Raw buffer resource over `bytes` bytes of global memory: a 4-byte load at an offset at
 // or past the end returns zero.  The builtins exist only in the device pass.
@@ -64,7 +64,15 @@ __device__ inline vbuf make_vbuf(const float* base, int bytes) {
 __device__ inline float vbuf_load(vbuf rs, unsigned off) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, 0));
 }
+// a + b in a vector register, opaque to the compiler, so that a constant part of the
+// sum is not moved into a load's immediate offset
+__device__ inline unsigned vadd_sv(int a, unsigned b) {
+    unsigned r;
+    asm("v_add_u32 %0, %1, %2" : "=v"(r) : "s"(a), "v"(b));
+    return r;
+}
 #else
+__device__ inline unsigned vadd_sv(int a, unsigned b) { return (unsigned)a + b; }
 struct vbuf {};
 __device__ inline vbuf make_vbuf(const float*, int) { return {}; }
 __device__ inline float vbuf_load(vbuf, unsigned) { return 0.0f; }
@@ -83,10 +91,6 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
     const int H = a.H, pitch = a.H + 4;
     float* myrows = rows_lds + (size_t)wv * pitch;
     for (int i = tid; i < kGWaves * pitch; i += kGThreads) rows_lds[i] = 0.0f;
-    // lane % n for n = 1..8, four bits each: this lane's residue pass for n passes
-    unsigned pass_bits = 0;
-#pragma unroll
-    for (int n = 1; n <= kFpMaxPasses; ++n) pass_bits |= (unsigned)(lane % n) << (4 * (n - 1));
     __syncthreads();
 
     const float4 g = pp[p];
@@ -181,7 +185,11 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
                                 lo = max(lo, a.z0);
                                 hi = min(hi, zend);
                                 if (lo <= hi) {
-                                    const int npass = (int)ceilf(2.0f * __builtin_amdgcn_rcpf(U) + 1e-3f);
+                                    // ceil(2/U) passes, rounded up to a power of two (a
+                                    // multiple of ceil(2/U) passes is as safe); U > 1/1.9
+                                    // keeps it at most 4
+                                    const int np0 = (int)ceilf(2.0f * __builtin_amdgcn_rcpf(U) + 1e-3f);
+                                    const int npass = np0 <= 1 ? 1 : (np0 == 2 ? 2 : 4);
                                     rec = make_int4(xi * a.ny + yi, __float_as_int(U), __float_as_int(wc),
                                                     (lo - zrb) | ((hi - zrb) << kFpZBits) | (npass << (2 * kFpZBits)));
                                 }
@@ -245,9 +253,15 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
                 it.npass = (unsigned)w >> (2 * kFpZBits);
                 return it;
             };
-            // this lane's voxel of the chunk starting at zc (relative to zrb)
+            // this lane's voxel of the chunk starting at zc (relative to zrb).  The byte
+            // offset is formed in one register per chunk: lanes before the interval get
+            // an offset that wraps to a huge value, which the resource's range check
+            // rejects.  With the chunk's distance in the load's immediate offset instead,
+            // a wrapped register offset plus the immediate was measured to read zeros
+            // where the sum lands at the start of the resource.
+            const unsigned lane4 = (unsigned)lane * (unsigned)sizeof(float);
             auto load = [&](const Item& it, int zc) -> float {
-                return vbuf_load(it.rs, (unsigned)(zc - it.lo + lane) * (unsigned)sizeof(float));
+                return vbuf_load(it.rs, vadd_sv((zc - it.lo) * (int)sizeof(float), lane4));
             };
             auto zworld = [&](int zc) { return (float)(zrb + zc) - a.zpre; };
             const int nzr = zend - zrb + 1;  // slices from the run base to the slab end
@@ -255,99 +269,82 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
                 __syncthreads();
                 continue;
             }
-            // Each wave walks its items in order through three slots: the voxels of the
-            // next two items are in flight while the current one is processed (the slots
+            // Items are (candidate, group of G consecutive 64-voxel chunks), in walk order,
+            // dealt to the four waves round-robin: item q = ci * ngrp + j goes to wave
+            // q mod 4.  The pass count is 1, 2 or 4, so lanes l and l' of any two chunks of
+            // an item share a pass only if their z differ by a nonzero multiple of it, and
+            // then their rows are at least two apart: a pass's read-modify-writes touch
+            // distinct rows.  Each wave walks its items through S slots: the voxels of the
+            // next S - 1 items are in flight while the current one is processed (the slots
             // are named, not rotated, so that no in-flight register is copied).
-            if (nzr >= 128 * kGWaves) {
-                // Wave wv takes the 64-voxel chunks wv, wv + 4, ... of every candidate, two
-                // at a time (their rows are far apart): items are (candidate, chunk pair).
-                const int span = 128 * kGWaves;  // z covered by one chunk pair of every wave
-                const int n_it = nzr > 64 * wv ? (nzr - 64 * wv + span - 1) / span : 0;
-                int ci_f = 0, k_f = 0;  // next item to fetch
-                struct Slot { Item it; int zc; float x0, x1; };
+            const int nch = (nzr + 63) / 64;
+            auto walk = [&](auto g_const) {
+                constexpr int G = decltype(g_const)::value;
+                constexpr int S = G == 1 ? 3 : 2;
+                const int ngrp = (nch + G - 1) / G;
+                const int dci = kGWaves / ngrp, dj = kGWaves - dci * ngrp;
+                int ci_f = 0, j_f = wv;  // next item to fetch
+                while (j_f >= ngrp) { j_f -= ngrp; ++ci_f; }
+                struct Slot { Item it; int zc; float x[G]; };
                 auto next = [&](Slot& s) -> bool {
                     const bool valid = ci_f < ncand;
                     s.it = item_of(ci_f, valid);
-                    s.zc = 64 * wv + k_f * span;
-                    s.x0 = load(s.it, s.zc);
-                    s.x1 = load(s.it, s.zc + 64 * kGWaves);
-                    if (++k_f == n_it) { k_f = 0; ++ci_f; }
-                    __builtin_amdgcn_sched_barrier(0);  // keep the slots' loads in walk order
-                    return valid;
-                };
-                auto process = [&](const Slot& s) {
-                    float ta0, tb0, ta1, tb1;
-                    const int i0 = tap(zworld(s.zc), s.x0, s.it.U, s.it.wc, ta0, tb0);
-                    const int i1 = tap(zworld(s.zc + 64 * kGWaves), s.x1, s.it.U, s.it.wc, ta1, tb1);
-                    const int pid = (int)((pass_bits >> (4 * (s.it.npass - 1))) & 15u);
-                    for (int ps = 0; ps < s.it.npass; ++ps) {
-                        if (pid == ps) {
-                            // loads before stores: the four rows are distinct
-                            const float a0 = myrows[i0], a1 = myrows[i0 + 1];
-                            const float b0 = myrows[i1], b1 = myrows[i1 + 1];
-                            myrows[i0] = a0 + ta0;
-                            myrows[i0 + 1] = a1 + tb0;
-                            myrows[i1] = b0 + ta1;
-                            myrows[i1 + 1] = b1 + tb1;
-                        }
-                        __builtin_amdgcn_wave_barrier();
-                    }
-                };
-                Slot sa, sb;
-                bool ha = next(sa), hb = next(sb);
-                while (ha) {
-                    process(sa);
-                    ha = next(sa);
-                    if (!hb) break;
-                    process(sb);
-                    hb = next(sb);
-                }
-            } else {
-                // Thin slabs: items (candidate, 64-voxel chunk) in walk order go to the waves
-                // round-robin, one chunk at a time, so that every wave has work.  Item
-                // q = ci * nch + k; wave wv takes q = wv, wv + 4, ...
-                const int nch = (nzr + 63) / 64;
-                const int dci = kGWaves / nch, dk = kGWaves - dci * nch;
-                int ci_f = 0, k_f = wv;
-                while (k_f >= nch) { k_f -= nch; ++ci_f; }
-                struct Slot { Item it; int zc; float x0; };
-                auto next = [&](Slot& s) -> bool {
-                    const bool valid = ci_f < ncand;
-                    s.it = item_of(ci_f, valid);
-                    s.zc = 64 * k_f;
-                    s.x0 = load(s.it, s.zc);
+                    s.zc = 64 * G * j_f;
+#pragma unroll
+                    for (int k = 0; k < G; ++k) s.x[k] = load(s.it, s.zc + 64 * k);
                     ci_f += dci;
-                    k_f += dk;
-                    if (k_f >= nch) { k_f -= nch; ++ci_f; }
+                    j_f += dj;
+                    if (j_f >= ngrp) { j_f -= ngrp; ++ci_f; }
                     __builtin_amdgcn_sched_barrier(0);  // keep the slots' loads in walk order
                     return valid;
                 };
                 auto process = [&](const Slot& s) {
-                    float ta0, tb0;
-                    const int i0 = tap(zworld(s.zc), s.x0, s.it.U, s.it.wc, ta0, tb0);
-                    const int pid = (int)((pass_bits >> (4 * (s.it.npass - 1))) & 15u);
+                    float ta[G], tb[G];
+                    int ix[G];
+#pragma unroll
+                    for (int k = 0; k < G; ++k) ix[k] = tap(zworld(s.zc + 64 * k), s.x[k], s.it.U, s.it.wc, ta[k], tb[k]);
+                    const int pid = lane & (s.it.npass - 1);
                     for (int ps = 0; ps < s.it.npass; ++ps) {
                         if (pid == ps) {
-                            const float a0 = myrows[i0], a1 = myrows[i0 + 1];
-                            myrows[i0] = a0 + ta0;
-                            myrows[i0 + 1] = a1 + tb0;
+                            // loads before stores: the 2G rows are distinct
+                            float r0[G], r1[G];
+#pragma unroll
+                            for (int k = 0; k < G; ++k) { r0[k] = myrows[ix[k]]; r1[k] = myrows[ix[k] + 1]; }
+#pragma unroll
+                            for (int k = 0; k < G; ++k) { myrows[ix[k]] = r0[k] + ta[k]; myrows[ix[k] + 1] = r1[k] + tb[k]; }
                         }
                         __builtin_amdgcn_wave_barrier();
                     }
                 };
-                Slot sa, sb, sc;
-                bool ha = next(sa), hb = next(sb), hc = next(sc);
-                while (ha) {
-                    process(sa);
-                    ha = next(sa);
-                    if (!hb) break;
-                    process(sb);
-                    hb = next(sb);
-                    if (!hc) break;
-                    process(sc);
-                    hc = next(sc);
+                if constexpr (S == 3) {
+                    Slot sa, sb, sc;
+                    bool ha = next(sa), hb = next(sb), hc = next(sc);
+                    while (ha) {
+                        process(sa);
+                        ha = next(sa);
+                        if (!hb) break;
+                        process(sb);
+                        hb = next(sb);
+                        if (!hc) break;
+                        process(sc);
+                        hc = next(sc);
+                    }
+                } else {
+                    Slot sa, sb;
+                    bool ha = next(sa), hb = next(sb);
+                    while (ha) {
+                        process(sa);
+                        ha = next(sa);
+                        if (!hb) break;
+                        process(sb);
+                        hb = next(sb);
+                    }
                 }
-            }
+            };
+            if (nch >= 4) walk(std::integral_constant<int, 4>{});
+            else if (nch == 3) walk(std::integral_constant<int, 3>{});
+            else if (nch == 2) walk(std::integral_constant<int, 2>{});
+            else walk(std::integral_constant<int, 1>{});
             __syncthreads();
         }
     }

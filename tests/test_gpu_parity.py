@@ -288,11 +288,14 @@ def test_forward_projector_matches_fp64_oracle(gpu, oracle):
     assert err < 1e-5
 
 
-@pytest.mark.parametrize("nz", [40, 520])
+@pytest.mark.parametrize("nz", [40, 100, 192, 256, 520])
 def test_forward_projector_wide_cone_and_slab_widths(gpu, oracle, nz):
     """The widest cone the geometry allows (FCD just above the detector width) makes the
-    candidate wedge of a detector column widest; a 40-slice volume takes the projector's
-    thin-slab walk and a 520-slice one its paired-chunk walk (partial last chunks)."""
+    candidate wedge of a detector column widest.  The slab depths take the projector's
+    walk with one, two, three and four 64-slice chunks per item, and several items per
+    candidate (520 slices: partial last chunks).  With 48 detector rows most of a voxel
+    column lies outside the window, so chunk loads start before the candidate's z
+    interval (the case of wrapped buffer offsets)."""
     import torch
     import ct_amd
     W, H, P = 64, 48, 10
