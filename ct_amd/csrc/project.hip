@@ -42,14 +42,15 @@ struct FpArgs {
 //    backprojection's anchored row index, and add the two row taps of each voxel into
 //    the wave's own row accumulators in LDS.  Chunks start at multiples of 32 in grid z,
 //    so a lane's row anchor and its offset from it are fixed per lane.  Rows of voxels k
-//    slices apart differ by at least k*U - 1, so the lanes are split by residue into
-//    ceil(2/U) passes, rounded up to 2 or 4 (which divide the chunk length), whose taps
-//    never share a row; the read-modify-writes need no atomics.
+//    slices apart differ by at least k*U - 1, so lanes are split by residue modulo
+//    ceil(2/U), rounded up to 2 or 4 (which divide the chunk length), between two row
+//    copies per wave and, for 4, two passes: taps that meet never share a row, and the
+//    read-modify-writes need no atomics.
 //  - At the end the four waves' rows are summed into the image.
 // Every (voxel, tap) pair is added exactly once, and the order of all additions is fixed:
 // the result does not depend on scheduling.
 constexpr int kGThreads = 256, kGWaves = kGThreads / 64;
-constexpr int kSlots = 1024;  // candidate slots per pass (16 B each)
+constexpr int kSlots = 512;  // candidate slots per pass (16 B each)
 constexpr int kMaxRowsFp = 4096;  // detector rows held in LDS per wave
 constexpr int kFpZBits = 14;      // z interval bounds in a candidate record (relative to the run base)
 constexpr int kFpMaxSlab = (1 << kFpZBits) - 2 * kRowAnchor;  // slices per launch
@@ -83,14 +84,17 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
                                                               float* __restrict__ proj) {
     __shared__ int4 slots[kSlots];
     __shared__ int wave_cnt[kGWaves];
-    extern __shared__ float rows_lds[];  // kGWaves x (H + 4): row r at index r + 2; rows -2, -1, H, H + 1 absorb taps off the image
+    // kGWaves x 2 copies of (H + 4) rows: row r at index r + 2; rows -2, -1, H, H + 1
+    // absorb taps off the image.  Even lanes of a wave add into its first copy, odd
+    // lanes into its second.
+    extern __shared__ float rows_lds[];
     const int c = blockIdx.x;
     const int p = blockIdx.y;
     const int tid = (int)threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int H = a.H, pitch = a.H + 4;
-    float* myrows = rows_lds + (size_t)wv * pitch;
-    for (int i = tid; i < kGWaves * pitch; i += kGThreads) rows_lds[i] = 0.0f;
+    float* myrows = rows_lds + (size_t)(2 * wv + (lane & 1)) * pitch;
+    for (int i = tid; i < 2 * kGWaves * pitch; i += kGThreads) rows_lds[i] = 0.0f;
     __syncthreads();
 
     const float4 g = pp[p];
@@ -271,12 +275,15 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
             }
             // Items are (candidate, group of G consecutive 64-voxel chunks), in walk order,
             // dealt to the four waves round-robin: item q = ci * ngrp + j goes to wave
-            // q mod 4.  The pass count is 1, 2 or 4, so lanes l and l' of any two chunks of
-            // an item share a pass only if their z differ by a nonzero multiple of it, and
-            // then their rows are at least two apart: a pass's read-modify-writes touch
-            // distinct rows.  Each wave walks its items through S slots: the voxels of the
-            // next S - 1 items are in flight while the current one is processed (the slots
-            // are named, not rotated, so that no in-flight register is copied).
+            // q mod 4.  With n = ceil(2/U) rounded up to 1, 2 or 4, voxels n or more slices
+            // apart have rows at least two apart.  Lanes of one parity add into one of the
+            // wave's two row copies, and for n = 4 in two passes by (lane >> 1) & 1: lanes
+            // l and l' of any two chunks of an item (chunks are 64 slices apart) meet in a
+            // copy and pass only if their z differ by a nonzero multiple of 2 (n <= 2) or
+            // 4, so a pass's read-modify-writes touch distinct rows.  Each wave walks its
+            // items through S slots: the voxels of the next S - 1 items are in flight
+            // while the current one is processed (the slots are named, not rotated, so that
+            // no in-flight register is copied).
             const int nch = (nzr + 63) / 64;
             auto walk = [&](auto g_const) {
                 constexpr int G = decltype(g_const)::value;
@@ -303,17 +310,22 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
                     int ix[G];
 #pragma unroll
                     for (int k = 0; k < G; ++k) ix[k] = tap(zworld(s.zc + 64 * k), s.x[k], s.it.U, s.it.wc, ta[k], tb[k]);
-                    const int pid = lane & (s.it.npass - 1);
-                    for (int ps = 0; ps < s.it.npass; ++ps) {
-                        if (pid == ps) {
-                            // loads before stores: the 2G rows are distinct
-                            float r0[G], r1[G];
+                    // loads before stores: the 2G rows of a pass are distinct
+                    auto rmw = [&]() {
+                        float r0[G], r1[G];
 #pragma unroll
-                            for (int k = 0; k < G; ++k) { r0[k] = myrows[ix[k]]; r1[k] = myrows[ix[k] + 1]; }
+                        for (int k = 0; k < G; ++k) { r0[k] = myrows[ix[k]]; r1[k] = myrows[ix[k] + 1]; }
 #pragma unroll
-                            for (int k = 0; k < G; ++k) { myrows[ix[k]] = r0[k] + ta[k]; myrows[ix[k] + 1] = r1[k] + tb[k]; }
+                        for (int k = 0; k < G; ++k) { myrows[ix[k]] = r0[k] + ta[k]; myrows[ix[k] + 1] = r1[k] + tb[k]; }
+                    };
+                    if (s.it.npass <= 2) {
+                        rmw();  // the lane parity picks the copy: one pass
+                    } else {
+                        const int pid = (lane >> 1) & 1;
+                        for (int ps = 0; ps < 2; ++ps) {
+                            if (pid == ps) rmw();
+                            __builtin_amdgcn_wave_barrier();
                         }
-                        __builtin_amdgcn_wave_barrier();
                     }
                 };
                 if constexpr (S == 3) {
@@ -348,10 +360,10 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
             __syncthreads();
         }
     }
-    // rows of the four waves, summed in wave order, into the image
+    // both row copies of the four waves, summed in a fixed order, into the image
     for (int r = tid; r < H; r += kGThreads) {
         float sum = rows_lds[r + 2];
-        for (int w = 1; w < kGWaves; ++w) sum += rows_lds[(size_t)w * pitch + r + 2];
+        for (int w = 1; w < 2 * kGWaves; ++w) sum += rows_lds[(size_t)w * pitch + r + 2];
         proj[((size_t)p * H + r) * a.W + c] += sum;
     }
 }
@@ -498,7 +510,7 @@ ctg_status launch_forward_project(const ctg_geometry& g, const float4* d_params,
         return fail(CTG_ERROR_UNSUPPORTED, "forward projection needs the reconstruction cylinder within 0.9 FCD");
     dim3 grid((unsigned)a.W, (unsigned)nproj);
     if (grid.y > 65535u) return fail(CTG_ERROR_UNSUPPORTED, "forward projection grid too large");
-    const size_t lds = sizeof(float) * (size_t)kGWaves * (a.H + 4);
+    const size_t lds = sizeof(float) * (size_t)(2 * kGWaves) * (a.H + 4);
     // the candidate records hold z relative to the launch's run base in kFpZBits bits
     for (int s = 0; s < nz; s += kFpMaxSlab) {
         a.z0 = z0 + s;
