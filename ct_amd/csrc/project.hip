@@ -230,7 +230,7 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
                 const float val = wc * xv;
                 tb = fv * val;
                 ta = val - tb;
-                return (xv != 0.0f ? min(max(v0, -2), H) : H) + 2;
+                return (xv != 0.0f ? v0 : H) + 2;  // in-window rows are 0 .. H - 2 (checked on the host)
             };
             // a candidate record: column, U, weight, and z interval [lo, hi] relative to
             // zrb with the residue pass count in the top bits.  The column's voxels in
@@ -508,6 +508,11 @@ ctg_status launch_forward_project(const ctg_geometry& g, const float4* d_params,
     // U = FCD / (FCD - s) with |s| < radius stays in (1/1.9, 10): at most 4 residue passes
     if (!(g.radius < 0.9f * g.fcd))
         return fail(CTG_ERROR_UNSUPPORTED, "forward projection needs the reconstruction cylinder within 0.9 FCD");
+    // a voxel inside the v window reads rows v0, v0 + 1 with 0 <= v0 <= H - 2 (vIndex =
+    // vPre - v within about 1e-3 of rounding); the kernel indexes its row accumulators
+    // with v0 unclamped
+    if (!(g.v_pre - g.v_hi >= -0.5f && g.v_pre - g.v_lo <= (float)g.det_height - 1.05f))
+        return fail(CTG_ERROR_UNSUPPORTED, "forward projection needs the v window inside the detector rows");
     dim3 grid((unsigned)a.W, (unsigned)nproj);
     if (grid.y > 65535u) return fail(CTG_ERROR_UNSUPPORTED, "forward projection grid too large");
     const size_t lds = sizeof(float) * (size_t)(2 * kGWaves) * (a.H + 4);

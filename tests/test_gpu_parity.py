@@ -288,6 +288,39 @@ def test_forward_projector_matches_fp64_oracle(gpu, oracle):
     assert err < 1e-5
 
 
+def test_forward_projector_rejects_geometry_outside_its_bounds(gpu):
+    """The projector's row accumulators and residue passes rely on the reconstruction
+    cylinder lying well inside the source circle and on the v window lying inside the
+    detector rows (both hold for every geometry the CtVolume rules produce); other
+    geometries are refused before any kernel runs."""
+    import ctypes
+    import torch
+    import ct_amd
+    from ct_amd import _ctg
+    W, H, P = 32, 24, 2
+    base = ct_amd.geometry_grid(W, H, 200.0, 0.0, 16, 16, 8)
+    x = torch.zeros((16, 16, 8), dtype=torch.float32, device="cuda")
+    out = torch.zeros((P, H, W), dtype=torch.float32, device="cuda")
+    for field, value in (("radius", 0.95 * base.fcd), ("v_lo", base.v_pre - H + 0.5),
+                         ("v_hi", base.v_pre + 1.0)):
+        g = _ctg.Geometry()
+        ctypes.memmove(ctypes.byref(g), ctypes.byref(base), ctypes.sizeof(g))
+        setattr(g, field, value)
+        ctx = ct_amd.Context(0)
+        ctx.set_geometry(g)
+        ctx.set_projections(np.array([0.0, 90.0]))
+        with pytest.raises(_ctg.CtgError) as e:
+            ctx.forward_project(x, 0, 8, 0, P, out)
+        assert e.value.status == _ctg.ERROR_UNSUPPORTED, field
+    # the unmodified geometry projects
+    ctx = ct_amd.Context(0)
+    ctx.set_geometry(base)
+    ctx.set_projections(np.array([0.0, 90.0]))
+    ctx.forward_project(x, 0, 8, 0, P, out)
+    torch.cuda.synchronize()
+    assert float(out.abs().max()) == 0.0
+
+
 @pytest.mark.parametrize("nz", [40, 100, 192, 256, 520])
 def test_forward_projector_wide_cone_and_slab_widths(gpu, oracle, nz):
     """The widest cone the geometry allows (FCD just above the detector width) makes the
