@@ -320,8 +320,8 @@ ctg_status ctg_destroy(ctg_context* ctx) {
     (void)hipDeviceSynchronize();
     ctx->proj_dev.release();
     ctx->proj_dev64.release();
-    ctx->ftab.resp_f.release(); ctx->ftab.tw_f.release(); ctx->ftab.conv_f.release();
-    ctx->ftab.resp_d.release(); ctx->ftab.tw_d.release(); ctx->ftab.conv_d.release();
+    ctx->ftab.resp_f.release(); ctx->ftab.tw_f.release(); ctx->ftab.pad_f.release();
+    ctx->ftab.resp_d.release(); ctx->ftab.tw_d.release(); ctx->ftab.pad_d.release();
     ctx->rows_f32.release();
     ctx->rows_f64.release();
     ctx->maxabs.release();

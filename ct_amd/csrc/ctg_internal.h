@@ -69,8 +69,8 @@ struct DevBuf {
 struct FilterTables {
     int W = -1;
     int filter = -1;
-    DevBuf resp_f, tw_f, conv_f;
-    DevBuf resp_d, tw_d, conv_d;
+    DevBuf resp_f, tw_f, pad_f;  // pad: transform of the padded kernel (widths that are not powers of two)
+    DevBuf resp_d, tw_d, pad_d;
 };
 
 }  // namespace ctg

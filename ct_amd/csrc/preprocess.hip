@@ -15,12 +15,17 @@
 //  - Other powers of two, and the FP64 yardstick: one workgroup per row pair, the pair
 //    in LDS, a radix-2 DIT forward FFT (bit-reversed load), the response and a
 //    conjugation, then a DIF FFT whose bit-reversed output is read back in natural order.
-//  - Other W: a direct circular convolution with the precomputed kernel.
+//  - Other W: the row's circular convolution with the filter kernel g (length W) is
+//    the middle of a linear convolution with g extended periodically to 2W - 1 taps,
+//    done in a power-of-two transform of length M >= 2W - 1 with the LDS radix-2 pair
+//    (no wrap-around reaches the W outputs kept).
 #include "ctg_internal.h"
 
 #include <hip/hip_fp16.h>
 
 #include <cmath>
+#include <complex>
+#include <vector>
 
 namespace ctg {
 namespace {
@@ -40,6 +45,7 @@ struct PreArgs {
     double dla, dlb, dlk;
     float fcd, upre, vpre;
     int do_log, do_weight, do_filter, pow2;
+    int M, log2M;  // transform length: W for powers of two, else the padded length
 };
 
 __device__ inline float dlog(float x) { return logf(x); }
@@ -131,12 +137,12 @@ __global__ __launch_bounds__(kRowThreads) void rows_kernel(PreArgs a, const Tin*
                                                           const float* __restrict__ lin,
                                                           const T* __restrict__ resp,
                                                           const Cx<T>* __restrict__ tw,
-                                                          const T* __restrict__ conv,
+                                                          const Cx<T>* __restrict__ pad_resp,
                                                           T* __restrict__ out,
                                                           unsigned* __restrict__ maxabs) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* re = reinterpret_cast<T*>(smem_raw);
-    T* im = re + a.W;
+    T* im = re + a.M;
     const int W = a.W;
     const long long ra = 2ll * blockIdx.x;
     const long long rb = ra + 1;
@@ -161,30 +167,28 @@ __global__ __launch_bounds__(kRowThreads) void rows_kernel(PreArgs a, const Tin*
         }
         fft_dif<T>(re, im, W, a.log2W, tw);
     } else if (a.do_filter) {
-        T* g = im + W;
-        for (int n = threadIdx.x; n < W; n += blockDim.x) {
-            re[n] = load_pixel<T, Tin>(raw, lin, offa + n, a, rin_a, n);
-            im[n] = hasb ? load_pixel<T, Tin>(raw, lin, offb + n, a, rin_b, n) : (T)0;
-            g[n] = conv[n];
+        // y[n] = sum_m x[m] g[(n - m) mod W] = (x * h)[n + W - 1] with h[j] = g[(j - W + 1)
+        // mod W], j < 2W - 1: x zero-padded to M, times the M-point transform of h (over M,
+        // precomputed), the same conjugate trick, outputs read at n + W - 1
+        const int M = a.M;
+        for (int n = threadIdx.x; n < M; n += blockDim.x) {
+            T xa = (T)0, xb = (T)0;
+            if (n < W) {
+                xa = load_pixel<T, Tin>(raw, lin, offa + n, a, rin_a, n);
+                if (hasb) xb = load_pixel<T, Tin>(raw, lin, offb + n, a, rin_b, n);
+            }
+            unsigned pos = bitrev((unsigned)n, a.log2M);
+            re[pos] = xa;
+            im[pos] = xb;
         }
-        __syncthreads();
-        // direct circular convolution; results go to registers, then back to LDS
-        T ya[8], yb[8];
-        int cnt = 0;
-        for (int n = threadIdx.x; n < W && cnt < 8; n += blockDim.x, ++cnt) {
-            T sa = 0, sb = 0;
-            for (int m = 0; m <= n; ++m) { T gk = g[n - m]; sa += re[m] * gk; sb += im[m] * gk; }
-            for (int m = n + 1; m < W; ++m) { T gk = g[n - m + W]; sa += re[m] * gk; sb += im[m] * gk; }
-            ya[cnt] = sa;
-            yb[cnt] = sb;
+        fft_dit<T>(re, im, M, a.log2M, tw);
+        for (int k = threadIdx.x; k < M; k += blockDim.x) {
+            const Cx<T> h = pad_resp[k];
+            const T xr = re[k], xi = im[k];
+            re[k] = xr * h.re - xi * h.im;
+            im[k] = -(xr * h.im + xi * h.re);
         }
-        __syncthreads();
-        cnt = 0;
-        for (int n = threadIdx.x; n < W && cnt < 8; n += blockDim.x, ++cnt) {
-            re[n] = ya[cnt];
-            im[n] = -yb[cnt];  // read back with the same sign convention as the FFT path
-        }
-        __syncthreads();
+        fft_dif<T>(re, im, M, a.log2M, tw);
     } else {
         for (int n = threadIdx.x; n < W; n += blockDim.x) {
             re[n] = load_pixel<T, Tin>(raw, lin, offa + n, a, rin_a, n);
@@ -193,10 +197,14 @@ __global__ __launch_bounds__(kRowThreads) void rows_kernel(PreArgs a, const Tin*
         __syncthreads();
     }
 
-    const bool bitrev_out = a.do_filter && a.pow2;
+    // filtered outputs sit in bit-reversed order (at n + W - 1 for padded widths)
+    const int out_shift = a.pow2 ? 0 : W - 1;
+    auto out_pos = [&](int n) -> unsigned {
+        return a.do_filter ? bitrev((unsigned)(n + out_shift), a.log2M) : (unsigned)n;
+    };
     float mloc = 0.0f;
     for (int n = threadIdx.x; n < W; n += blockDim.x) {
-        unsigned pos = bitrev_out ? bitrev((unsigned)n, a.log2W) : (unsigned)n;
+        unsigned pos = out_pos(n);
         T ya = re[pos];
         T yb = -im[pos];
         out[offa + n] = ya;
@@ -216,7 +224,7 @@ __global__ __launch_bounds__(kRowThreads) void rows_kernel(PreArgs a, const Tin*
             } else {
                 float ma = 0.0f, mb = 0.0f;
                 for (int n = threadIdx.x; n < W; n += blockDim.x) {
-                    unsigned pos = bitrev_out ? bitrev((unsigned)n, a.log2W) : (unsigned)n;
+                    unsigned pos = out_pos(n);
                     ma = fmaxf(ma, fabsf((float)re[pos]));
                     mb = fmaxf(mb, fabsf((float)im[pos]));
                 }
@@ -562,6 +570,28 @@ int ilog2_exact(int W) {
     return (1 << l) == W ? l : -1;
 }
 
+// Transform length of the row filter: W itself for powers of two, otherwise the
+// smallest power of two >= 2W - 1 (the padded linear convolution).
+int filter_log2_length(int W) {
+    const int l = ilog2_exact(W);
+    if (l >= 0) return l;
+    int m = 0;
+    while ((1 << m) < 2 * W - 1) ++m;
+    return m;
+}
+
+void set_transform(PreArgs& a) {
+    a.log2W = ilog2_exact(a.W);
+    a.pow2 = a.log2W >= 0;
+    if (a.do_filter) {
+        a.log2M = filter_log2_length(a.W);
+        a.M = 1 << a.log2M;
+    } else {
+        a.log2M = a.log2W;
+        a.M = a.W;
+    }
+}
+
 void fill_log_params(PreArgs& a, ctg_pixel_type pixel, float base, ctg_log_form form) {
     const double B = base;
     const double S = pixel == CTG_PIXEL_U8 ? 255.0 : (pixel == CTG_PIXEL_U16 ? 65535.0 : 1.0);
@@ -614,11 +644,10 @@ ctg_status launch_rows(ctg_context* ctx, const void* raw, ctg_pixel_type pixel, 
     const FilterTables& ft = ctx->ftab;
     const T* resp = sizeof(T) == 4 ? (const T*)ft.resp_f.ptr : (const T*)ft.resp_d.ptr;
     const Cx<T>* tw = sizeof(T) == 4 ? (const Cx<T>*)ft.tw_f.ptr : (const Cx<T>*)ft.tw_d.ptr;
-    const T* conv = sizeof(T) == 4 ? (const T*)ft.conv_f.ptr : (const T*)ft.conv_d.ptr;
-    size_t lds = (size_t)a.W * sizeof(T) * ((a.do_filter && !a.pow2) ? 3 : 2);
-    if (lds > 160 * 1024) return fail(CTG_ERROR_UNSUPPORTED, "detector width %d exceeds the LDS row buffer", a.W);
-    if (a.do_filter && !a.pow2 && a.W > 8 * kRowThreads)
-        return fail(CTG_ERROR_UNSUPPORTED, "non power-of-two detector width %d > %d", a.W, 8 * kRowThreads);
+    const Cx<T>* pad = sizeof(T) == 4 ? (const Cx<T>*)ft.pad_f.ptr : (const Cx<T>*)ft.pad_d.ptr;
+    const size_t lds = (size_t)a.M * sizeof(T) * 2;
+    if (lds > 160 * 1024)
+        return fail(CTG_ERROR_UNSUPPORTED, "detector width %d (transform length %d) exceeds the LDS row buffer", a.W, a.M);
     if constexpr (sizeof(T) == 4) {
         if (a.do_filter && a.pow2 && a.W >= 256 && a.W <= 2048) {
             switch (a.W) {
@@ -632,16 +661,16 @@ ctg_status launch_rows(ctg_context* ctx, const void* raw, ctg_pixel_type pixel, 
     dim3 grid((unsigned)((nrows + 1) / 2));
     if (raw == nullptr) {
         hipLaunchKernelGGL((rows_kernel<T, float>), grid, dim3(kRowThreads), lds, st, a, (const float*)nullptr,
-                           lin, resp, tw, conv, out, maxabs);
+                           lin, resp, tw, pad, out, maxabs);
     } else if (pixel == CTG_PIXEL_U8) {
         hipLaunchKernelGGL((rows_kernel<T, uint8_t>), grid, dim3(kRowThreads), lds, st, a, (const uint8_t*)raw,
-                           lin, resp, tw, conv, out, maxabs);
+                           lin, resp, tw, pad, out, maxabs);
     } else if (pixel == CTG_PIXEL_U16) {
         hipLaunchKernelGGL((rows_kernel<T, uint16_t>), grid, dim3(kRowThreads), lds, st, a,
-                           (const uint16_t*)raw, lin, resp, tw, conv, out, maxabs);
+                           (const uint16_t*)raw, lin, resp, tw, pad, out, maxabs);
     } else {
         hipLaunchKernelGGL((rows_kernel<T, float>), grid, dim3(kRowThreads), lds, st, a, (const float*)raw,
-                           lin, resp, tw, conv, out, maxabs);
+                           lin, resp, tw, pad, out, maxabs);
     }
     CTG_HIP(hipGetLastError());
     return CTG_OK;
@@ -657,8 +686,6 @@ ctg_status launch_preprocess_rows(ctg_context* ctx, const void* d_raw, ctg_pixel
     PreArgs a{};
     a.W = g.det_width;
     a.H = g.det_height;
-    a.log2W = ilog2_exact(a.W);
-    a.pow2 = a.log2W >= 0;
     a.nrows = (long long)nproj * a.H;
     fill_log_params(a, pixel, base, form);
     a.fcd = g.fcd;
@@ -667,6 +694,7 @@ ctg_status launch_preprocess_rows(ctg_context* ctx, const void* d_raw, ctg_pixel
     a.do_log = do_log;
     a.do_weight = do_weight;
     a.do_filter = do_filter;
+    set_transform(a);
     if (do_filter) CTG_TRY(ensure_filter_tables(ctx, a.W, filter));
     return launch_rows<float>(ctx, do_log ? d_raw : nullptr, pixel, do_log ? nullptr : (const float*)d_raw,
                               a.nrows, a, d_out, d_maxabs, st);
@@ -678,10 +706,9 @@ ctg_status launch_filter_rows_only(ctg_context* ctx, const float* d_in, float* d
     PreArgs a{};
     a.W = g.det_width;
     a.H = g.det_height > 0 ? g.det_height : 1;
-    a.log2W = ilog2_exact(a.W);
-    a.pow2 = a.log2W >= 0;
     a.nrows = rows;
     a.do_filter = 1;
+    set_transform(a);
     CTG_TRY(ensure_filter_tables(ctx, a.W, filter));
     return launch_rows<float>(ctx, nullptr, CTG_PIXEL_F32, d_in, rows, a, d_out, nullptr, st);
 }
@@ -692,8 +719,6 @@ ctg_status launch_preprocess_rows_f64(ctg_context* ctx, const void* d_raw, ctg_p
     PreArgs a{};
     a.W = g.det_width;
     a.H = g.det_height;
-    a.log2W = ilog2_exact(a.W);
-    a.pow2 = a.log2W >= 0;
     a.nrows = (long long)nproj * a.H;
     fill_log_params(a, pixel, base, CTG_LOG_CPU_FORM);
     a.fcd = g.fcd;
@@ -702,6 +727,7 @@ ctg_status launch_preprocess_rows_f64(ctg_context* ctx, const void* d_raw, ctg_p
     a.do_log = 1;
     a.do_weight = 1;
     a.do_filter = 1;
+    set_transform(a);
     CTG_TRY(ensure_filter_tables(ctx, a.W, filter));
     return launch_rows<double>(ctx, d_raw, pixel, nullptr, a.nrows, a, d_out, nullptr, st);
 }
@@ -724,8 +750,10 @@ ctg_status launch_layout(const ctg_geometry& g, const float* d_rows, int nproj, 
 }
 
 // Host-side tables in double: window H(k) with the float/double evaluation of
-// Ct/CtVolume.cpp:868-885 (N = W/2+1), its real even extension, twiddles
-// exp(-2 pi i m / W), and the circular-convolution kernel for other W.
+// Ct/CtVolume.cpp:868-885 (N = W/2+1) and its real even extension; twiddles
+// exp(-2 pi i m / M) for the transform length M; for other W the circular-convolution
+// kernel g[m] = sum_k H(k) cos(2 pi k m / W), extended periodically to 2W - 1 taps,
+// and its M-point transform over M.
 ctg_status ensure_filter_tables(ctg_context* ctx, int W, int filter) {
     FilterTables& ft = ctx->ftab;
     if (ft.W == W && ft.filter == filter && ft.resp_f.ptr) return CTG_OK;
@@ -749,37 +777,66 @@ ctg_status ensure_filter_tables(ctg_context* ctx, int W, int filter) {
         }
         h[k] = hv;
     }
-    std::vector<double> resp(W), conv(W);
+    std::vector<double> resp(W);
     for (int k = 0; k < W; ++k) resp[k] = h[k <= W / 2 ? k : W - k];
-    std::vector<double> tw(2 * (size_t)(W / 2 > 0 ? W / 2 : 1));
-    for (int m = 0; m < W / 2; ++m) {
-        double ang = 2.0 * kPi * (double)m / (double)W;
+    const bool pow2 = ilog2_exact(W) >= 0;
+    const int log2M = filter_log2_length(W), M = 1 << log2M;
+    std::vector<double> tw(2 * (size_t)(M / 2 > 0 ? M / 2 : 1));
+    for (int m = 0; m < M / 2; ++m) {
+        double ang = 2.0 * kPi * (double)m / (double)M;
         tw[2 * m] = std::cos(ang);
         tw[2 * m + 1] = -std::sin(ang);
     }
-    if (ilog2_exact(W) < 0) {
+    std::vector<double> pad(2 * (size_t)M, 0.0);  // complex, interleaved
+    if (!pow2) {
+        std::vector<double> cosw(W), g(W);
+        for (int j = 0; j < W; ++j) cosw[j] = std::cos(2.0 * kPi * (double)j / (double)W);
         for (int m = 0; m < W; ++m) {
             double s = 0.0;
-            for (int k = 0; k < W; ++k)
-                s += resp[k] * std::cos(2.0 * kPi * (double)(((long long)k * m) % W) / (double)W);
-            conv[m] = s;
+            for (int k = 0; k < W; ++k) s += resp[k] * cosw[(int)(((long long)k * m) % W)];
+            g[m] = s;
+        }
+        std::vector<std::complex<double>> z(M);
+        for (int j = 0; j < 2 * W - 1; ++j) z[j] = g[((j - (W - 1)) % W + W) % W];
+        // iterative radix-2 transform exp(-2 pi i jk / M), over M
+        for (int i = 1, r = 0; i < M; ++i) {
+            int bit = M >> 1;
+            for (; r & bit; bit >>= 1) r ^= bit;
+            r ^= bit;
+            if (i < r) std::swap(z[i], z[r]);
+        }
+        for (int len = 2; len <= M; len <<= 1) {
+            for (int i0 = 0; i0 < M; i0 += len) {
+                for (int k = 0; k < len / 2; ++k) {
+                    const int t = k * (M / len);
+                    const std::complex<double> w(tw[2 * t], tw[2 * t + 1]);
+                    const std::complex<double> u = z[i0 + k], v = z[i0 + k + len / 2] * w;
+                    z[i0 + k] = u + v;
+                    z[i0 + k + len / 2] = u - v;
+                }
+            }
+        }
+        for (int k = 0; k < M; ++k) {
+            pad[2 * (size_t)k] = z[k].real() / (double)M;
+            pad[2 * (size_t)k + 1] = z[k].imag() / (double)M;
         }
     }
-    std::vector<float> resp_f(W), conv_f(W), tw_f(tw.size());
-    for (int i = 0; i < W; ++i) { resp_f[i] = (float)resp[i]; conv_f[i] = (float)conv[i]; }
+    std::vector<float> resp_f(W), tw_f(tw.size()), pad_f(pad.size());
+    for (int i = 0; i < W; ++i) resp_f[i] = (float)resp[i];
     for (size_t i = 0; i < tw.size(); ++i) tw_f[i] = (float)tw[i];
+    for (size_t i = 0; i < pad.size(); ++i) pad_f[i] = (float)pad[i];
     CTG_TRY(ft.resp_f.reserve(sizeof(float) * W));
-    CTG_TRY(ft.conv_f.reserve(sizeof(float) * W));
     CTG_TRY(ft.tw_f.reserve(sizeof(float) * tw_f.size()));
+    CTG_TRY(ft.pad_f.reserve(sizeof(float) * pad_f.size()));
     CTG_TRY(ft.resp_d.reserve(sizeof(double) * W));
-    CTG_TRY(ft.conv_d.reserve(sizeof(double) * W));
     CTG_TRY(ft.tw_d.reserve(sizeof(double) * tw.size()));
+    CTG_TRY(ft.pad_d.reserve(sizeof(double) * pad.size()));
     CTG_HIP(hipMemcpy(ft.resp_f.ptr, resp_f.data(), sizeof(float) * W, hipMemcpyHostToDevice));
-    CTG_HIP(hipMemcpy(ft.conv_f.ptr, conv_f.data(), sizeof(float) * W, hipMemcpyHostToDevice));
     CTG_HIP(hipMemcpy(ft.tw_f.ptr, tw_f.data(), sizeof(float) * tw_f.size(), hipMemcpyHostToDevice));
+    CTG_HIP(hipMemcpy(ft.pad_f.ptr, pad_f.data(), sizeof(float) * pad_f.size(), hipMemcpyHostToDevice));
     CTG_HIP(hipMemcpy(ft.resp_d.ptr, resp.data(), sizeof(double) * W, hipMemcpyHostToDevice));
-    CTG_HIP(hipMemcpy(ft.conv_d.ptr, conv.data(), sizeof(double) * W, hipMemcpyHostToDevice));
     CTG_HIP(hipMemcpy(ft.tw_d.ptr, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+    CTG_HIP(hipMemcpy(ft.pad_d.ptr, pad.data(), sizeof(double) * pad.size(), hipMemcpyHostToDevice));
     ft.W = W;
     ft.filter = filter;
     return CTG_OK;
