@@ -549,6 +549,7 @@ struct ReconPlan {
     int batch = 1;       // projections uploaded and preprocessed per step
     bool resident = true;
     int slab_nz = 0;
+    bool double_buffer = false;  // two device slabs: slab s downloads under slab s+1
 };
 
 // Memory plan of ctg_reconstruct: the working set (staging and FP32 rows of one batch),
@@ -583,6 +584,18 @@ ctg_status plan_reconstruct(ctg_context* ctx, ctg_pixel_type pixel, int nproj, c
     plan.slab_nz = (int)std::min<long long>(g.nz, for_slabs / slab_col);
     // split volumes use slabs of whole 32-slice runs (the backprojection's run length)
     if (plan.slab_nz < g.nz && plan.slab_nz >= 32) plan.slab_nz -= plan.slab_nz % 32;
+    // A resident volume of 256 or more slices that fits whole is cut into about four
+    // slabs of whole runs when two slab buffers and the converted copy fit: the
+    // download of each slab then runs under the backprojection of the next, and only
+    // the last slab's download is left at the end.
+    if (plan.resident && plan.slab_nz == g.nz && g.nz >= 256) {
+        const int part = (g.nz + 3) / 4;
+        const int pipe_nz = (part + 31) / 32 * 32;
+        if ((long long)col * 4 * 3 * pipe_nz <= for_slabs) {
+            plan.slab_nz = pipe_nz;
+            plan.double_buffer = true;
+        }
+    }
     return CTG_OK;
 }
 
@@ -621,34 +634,41 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
     const int batch = plan.batch;
     const bool resident = plan.resident;
     const int slab_nz = plan.slab_nz;
+    const bool dbl = plan.double_buffer;
     const long long filt = (long long)ctg_filtered_bytes(&g, store, nproj);
 
-    // Streams: uploads (s_up), preprocessing (s_pre), backprojection and downloads
-    // (s_cmp).  Staging buffer i (pinned host and device raw) carries batch b with
-    // i = b & 1: the host refills pinned[i] once upload b-2 is done, the upload of b
-    // waits until preprocessing b-2 has read raw_stage[i].
-    hipStream_t s_up = nullptr, s_pre = nullptr, s_cmp = nullptr;
+    // Streams: uploads (s_up), preprocessing (s_pre), backprojection (s_cmp) and
+    // downloads (s_dl).  Staging buffer i (pinned host and device raw) carries batch b
+    // with i = b & 1: the host refills pinned[i] once upload b-2 is done, the upload of
+    // b waits until preprocessing b-2 has read raw_stage[i].  With two slab buffers,
+    // slab s is backprojected into d_slab[s & 1] while slab s-1 is downloaded.
+    hipStream_t s_up = nullptr, s_pre = nullptr, s_cmp = nullptr, s_dl = nullptr;
     hipEvent_t ev_up[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr}, ev_dl[2] = {nullptr, nullptr},
-               ev_bp = nullptr;
+               ev_slab[2] = {nullptr, nullptr}, ev_bp = nullptr;
     void* pinned[2] = {nullptr, nullptr};
-    float* d_slab = nullptr;
+    void* pinned_dl[2] = {nullptr, nullptr};
+    float* d_slab[2] = {nullptr, nullptr};
     void* d_conv = nullptr;
     ctg_status st = CTG_OK;
     auto cleanup = [&]() {
         if (s_up) (void)hipStreamSynchronize(s_up);
         if (s_pre) (void)hipStreamSynchronize(s_pre);
         if (s_cmp) (void)hipStreamSynchronize(s_cmp);
+        if (s_dl) (void)hipStreamSynchronize(s_dl);
         for (int i = 0; i < 2; ++i) {
             if (ev_up[i]) (void)hipEventDestroy(ev_up[i]);
             if (ev_done[i]) (void)hipEventDestroy(ev_done[i]);
             if (ev_dl[i]) (void)hipEventDestroy(ev_dl[i]);
+            if (ev_slab[i]) (void)hipEventDestroy(ev_slab[i]);
             if (pinned[i]) (void)hipHostFree(pinned[i]);
+            if (pinned_dl[i]) (void)hipHostFree(pinned_dl[i]);
+            if (d_slab[i]) (void)hipFree(d_slab[i]);
         }
         if (ev_bp) (void)hipEventDestroy(ev_bp);
         if (s_up) (void)hipStreamDestroy(s_up);
         if (s_pre) (void)hipStreamDestroy(s_pre);
         if (s_cmp) (void)hipStreamDestroy(s_cmp);
-        if (d_slab) (void)hipFree(d_slab);
+        if (s_dl) (void)hipStreamDestroy(s_dl);
         if (d_conv) (void)hipFree(d_conv);
     };
 #define PIPE_HIP(call)                                                                                      \
@@ -670,24 +690,47 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
         }                         \
     } while (0)
 
-    // pinned staging: one batch of raw images, and at least 16 MiB for the downloads
-    const size_t stage_bytes = std::max(img * pb * batch, (size_t)16 << 20);
+    // pinned staging: one batch of raw images for the uploads, 32 MiB chunks for the
+    // downloads
+    const size_t stage_bytes = img * pb * batch;
+    const size_t dl_bytes = (size_t)32 << 20;
     PIPE_HIP(hipStreamCreateWithFlags(&s_up, hipStreamNonBlocking));
     PIPE_HIP(hipStreamCreateWithFlags(&s_pre, hipStreamNonBlocking));
     PIPE_HIP(hipStreamCreateWithFlags(&s_cmp, hipStreamNonBlocking));
+    PIPE_HIP(hipStreamCreateWithFlags(&s_dl, hipStreamNonBlocking));
     PIPE_HIP(hipEventCreateWithFlags(&ev_bp, hipEventDisableTiming));
     for (int i = 0; i < 2; ++i) {
         PIPE_HIP(hipEventCreateWithFlags(&ev_up[i], hipEventDisableTiming));
         PIPE_HIP(hipEventCreateWithFlags(&ev_done[i], hipEventDisableTiming));
         PIPE_HIP(hipEventCreateWithFlags(&ev_dl[i], hipEventDisableTiming));
+        PIPE_HIP(hipEventCreateWithFlags(&ev_slab[i], hipEventDisableTiming));
         PIPE_HIP(hipHostMalloc(&pinned[i], stage_bytes, hipHostMallocDefault));
+        PIPE_HIP(hipHostMalloc(&pinned_dl[i], dl_bytes, hipHostMallocDefault));
     }
     PIPE_TRY(ctx->raw_stage.reserve(img * pb * batch * 2));
     // resident: every projection's entries stay in HBM for all slabs; otherwise one
     // batch of entries is re-derived from its uploaded images for every slab
     PIPE_TRY(ctx->filtered.reserve((size_t)(resident ? filt : (long long)ctg_filtered_bytes(&g, store, batch))));
     PIPE_TRY(ctx->scales_tmp.reserve(sizeof(float) * nproj));
-    PIPE_HIP(hipMalloc((void**)&d_slab, col * 4 * (size_t)slab_nz));
+    // Z_FASTEST host volumes ([x][y][z]) take a cut planned along z as twice as many x
+    // ranges (whole 16-column tiles) instead: an x range is contiguous in the host
+    // volume, so its download is one sequential copy rather than a strided one per
+    // column, and the smaller last range leaves less download after the last
+    // backprojection (1024^3: 300 ms host to host against 310 ms for four ranges).  The backprojection of a range is the one of the whole grid restricted to
+    // it (a grid of its columns with x_pre shifted by its start): same voxels, same
+    // tiles, same results.
+    const int nslabs_z = (g.nz + slab_nz - 1) / slab_nz;
+    int part_x = g.nx;
+    if (dbl && order == CTG_Z_FASTEST) {
+        part_x = ((g.nx + 2 * nslabs_z - 1) / (2 * nslabs_z) + 15) / 16 * 16;
+        if ((size_t)part_x * g.ny * g.nz > col * (size_t)slab_nz) part_x = g.nx;  // no larger than the planned slab
+    }
+    const bool xcut = part_x < g.nx;
+    const size_t slab_floats = xcut ? (size_t)part_x * g.ny * g.nz : col * (size_t)slab_nz;
+    PIPE_HIP(hipMalloc((void**)&d_slab[0], slab_floats * 4));
+    if (dbl) PIPE_HIP(hipMalloc((void**)&d_slab[1], slab_floats * 4));
+    int cur = 0;                     // slab buffer being backprojected
+    ctg_geometry gcur = g;           // grid of the slab being backprojected
     if (order == CTG_X_FASTEST) PIPE_HIP(hipMalloc(&d_conv, col * 4 * (size_t)slab_nz));
     const int nb_total = (nproj + batch - 1) / batch;
 
@@ -715,14 +758,14 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
     auto backproject = [&](int p0, int np, int z0, int nz, int accumulate, int last_batch) -> ctg_status {
         if (last_batch >= 0) CTG_HIP(hipStreamWaitEvent(s_cmp, ev_done[last_batch & 1], 0));
         BpLaunch L{};
-        L.g = &g;
+        L.g = &gcur;
         L.d_params = ctx->proj_dev.as<float4>() + p0;
         L.h_params = ctx->proj_host.data() + p0;
         L.d_scales = ctx->scales_tmp.as<float>() + p0;
         L.d_filtered = (const char*)ctx->filtered.ptr + (resident ? (size_t)p0 * img * store_bytes(store) : 0);
         L.store = store;
         L.nproj = np;
-        L.d_volume = d_slab;
+        L.d_volume = d_slab[cur];
         L.z0 = z0;
         L.nz = nz;
         L.accumulate = accumulate;
@@ -730,27 +773,31 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
         CTG_HIP(hipEventRecord(ev_bp, s_cmp));
         return CTG_OK;
     };
-    // Slab -> host volume through the pinned staging buffers in chunks: the DMA of chunk
-    // k+1 runs while host threads place chunk k (contiguous for X_FASTEST; for
-    // Z_FASTEST, columns of nz values at a stride of nz_total).  Every earlier GPU step
-    // has finished (the first DMA follows the slab's backprojection on s_cmp, and the
-    // uploads that used the staging buffers precede it).
-    auto download = [&](int z0, int nz) -> ctg_status {
-        const float* src = d_slab;
+    // Slab buffer `buf` -> host volume on s_dl, through the pinned download buffers in
+    // chunks: the DMA of chunk k+1 runs while host threads place chunk k (contiguous for
+    // X_FASTEST; for Z_FASTEST, columns of nz values at a stride of nz_total).  It
+    // starts once the slab's backprojection (ev_slab[buf] on s_cmp) has finished and
+    // has finished itself when this returns.
+    auto download = [&](int z0, int nz, int x0, int nxs, int buf) -> ctg_status {
+        CTG_HIP(hipStreamWaitEvent(s_dl, ev_slab[buf], 0));
+        const float* src = d_slab[buf];
         if (order == CTG_X_FASTEST) {
-            CTG_TRY(launch_convert_volume(g, d_slab, nz, CTG_X_FASTEST, CTG_LITTLE_ENDIAN, d_conv, s_cmp));
+            CTG_TRY(launch_convert_volume(g, d_slab[buf], nz, CTG_X_FASTEST, CTG_LITTLE_ENDIAN, d_conv, s_dl));
             src = (const float*)d_conv;
         }
-        CTG_HIP(hipStreamSynchronize(s_cmp));
-        const bool zf = order == CTG_Z_FASTEST;
-        const size_t unit = zf ? (size_t)nz : 1;  // floats per unit: a column, or one value
-        const size_t n_units = zf ? col : col * (size_t)nz;
-        const size_t per_chunk = std::max<size_t>(1, stage_bytes / (unit * 4));
+        // an x range of a Z_FASTEST volume, or any X_FASTEST slab: one contiguous block
+        const bool contiguous = xcut || order == CTG_X_FASTEST;
+        const size_t block0 = xcut ? (size_t)x0 * g.ny * g.nz : (size_t)z0 * col;
+        const size_t unit = contiguous ? 1 : (size_t)nz;  // floats per unit: one value, or a column
+        const size_t n_units = xcut ? (size_t)nxs * g.ny * g.nz : (contiguous ? col * (size_t)nz : col);
+        const size_t per_chunk = std::max<size_t>(1, dl_bytes / (unit * 4));
+        if (per_chunk * unit * 4 > dl_bytes) return fail(CTG_ERROR_UNSUPPORTED, "volume column exceeds the download chunk");
         const size_t nchunks = (n_units + per_chunk - 1) / per_chunk;
         auto issue = [&](size_t k) -> ctg_status {
             const size_t u0 = k * per_chunk, u1 = std::min(n_units, u0 + per_chunk);
-            CTG_HIP(hipMemcpyAsync(pinned[k & 1], src + u0 * unit, (u1 - u0) * unit * 4, hipMemcpyDeviceToHost, s_cmp));
-            CTG_HIP(hipEventRecord(ev_dl[k & 1], s_cmp));
+            CTG_HIP(hipMemcpyAsync(pinned_dl[k & 1], src + u0 * unit, (u1 - u0) * unit * 4, hipMemcpyDeviceToHost,
+                                   s_dl));
+            CTG_HIP(hipEventRecord(ev_dl[k & 1], s_dl));
             return CTG_OK;
         };
         CTG_TRY(issue(0));
@@ -758,9 +805,9 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
             if (k + 1 < nchunks) CTG_TRY(issue(k + 1));
             CTG_HIP(hipEventSynchronize(ev_dl[k & 1]));
             const size_t u0 = k * per_chunk, u1 = std::min(n_units, u0 + per_chunk);
-            const float* stage = (const float*)pinned[k & 1];
-            if (!zf) {
-                parallel_memcpy(h_volume + (size_t)z0 * col + u0, stage, (u1 - u0) * 4);
+            const float* stage = (const float*)pinned_dl[k & 1];
+            if (contiguous) {
+                parallel_memcpy(h_volume + block0 + u0, stage, (u1 - u0) * 4);
             } else if (nz == g.nz) {
                 parallel_memcpy(h_volume + u0 * (size_t)g.nz, stage, (u1 - u0) * unit * 4);
             } else {
@@ -773,14 +820,28 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
         return CTG_OK;
     };
 
-    const int nslabs = (g.nz + slab_nz - 1) / slab_nz;
+    const int nslabs = xcut ? (g.nx + part_x - 1) / part_x : nslabs_z;
+    // slab s: z range [z0, z0 + nz) of x range [x0, x0 + nxs)
+    auto slab_of = [&](int s, int& z0, int& nz, int& x0, int& nxs) {
+        if (xcut) {
+            z0 = 0; nz = g.nz;
+            x0 = s * part_x; nxs = std::min(part_x, g.nx - x0);
+        } else {
+            z0 = s * slab_nz; nz = std::min(slab_nz, g.nz - z0);
+            x0 = 0; nxs = g.nx;
+        }
+    };
     for (int s = 0; s < nslabs; ++s) {
         if (ctx->cancel.load()) {
             st = fail(CTG_ERROR_INTERRUPTED, "interrupted");
             cleanup();
             return st;
         }
-        const int z0 = s * slab_nz, nz = std::min(slab_nz, g.nz - z0);
+        int z0, nz, x0, nxs;
+        slab_of(s, z0, nz, x0, nxs);
+        gcur = g;
+        gcur.nx = nxs;
+        gcur.x_pre = g.x_pre - (float)x0;
         if (resident) {
             // Every slab is backprojected in the same groups of batches (up to four), so
             // each voxel sees the same sequence of launches whatever the slab plan.  In
@@ -808,8 +869,27 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
                 PIPE_TRY(backproject(b0, nb, z0, nz, b > 0 ? 1 : 0, b));
             }
         }
-        PIPE_TRY(download(z0, nz));
-        if (progress) progress((double)(s + 1) / nslabs, user);
+        PIPE_HIP(hipEventRecord(ev_slab[cur], s_cmp));
+        if (!dbl) {
+            PIPE_TRY(download(z0, nz, x0, nxs, cur));
+            if (progress) progress((double)(s + 1) / nslabs, user);
+        } else {
+            // the previous slab downloads while this one is backprojected; the buffer it
+            // used is rewritten (slab s+1) only after that download has returned
+            if (s > 0) {
+                int zp, nzp, xp, nxp;
+                slab_of(s - 1, zp, nzp, xp, nxp);
+                PIPE_TRY(download(zp, nzp, xp, nxp, cur ^ 1));
+                if (progress) progress((double)s / nslabs, user);
+            }
+            cur ^= 1;
+        }
+    }
+    if (dbl) {
+        int zl, nzl, xl, nxl;
+        slab_of(nslabs - 1, zl, nzl, xl, nxl);
+        PIPE_TRY(download(zl, nzl, xl, nxl, cur ^ 1));
+        if (progress) progress(1.0, user);
     }
     cleanup();
 #undef PIPE_HIP

@@ -227,7 +227,11 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
 /* The plan ctg_reconstruct would follow for these inputs under the current memory
  * limits: *resident = 1 when every filtered projection stays in device memory for all
  * slabs (else each slab re-derives them batch by batch from the uploaded images), and
- * *slab_nz = z-slices per slab.  Fails with CTG_ERROR_INSUFFICIENT_VRAM, as the
+ * *slab_nz = z-slices per slab.  A resident volume of 256 or more slices that fits
+ * whole is still cut into four slabs when a second slab buffer fits, so that each
+ * slab's download runs under the next slab's backprojection (for Z_FASTEST output the
+ * cut runs along x, into twice as many ranges of whole 16-column tiles, which the
+ * host volume holds contiguously).  Fails with CTG_ERROR_INSUFFICIENT_VRAM, as the
  * reconstruction would, when no slab fits (getMaxChunkSize() == 0,
  * Ct/CtVolume.cpp:1120-1126); the reference's counterpart for callers sizing a job is
  * getRequiredMemoryUpperBound (Ct/CtVolume.cpp:469-480). */

@@ -440,6 +440,50 @@ def test_convert_volume_orders_and_byte_swap(gpu):
     assert np.array_equal(out.cpu().numpy().view(">f4").astype(np.float32), v.ravel())
 
 
+def test_reconstruct_double_buffered_slabs(gpu):
+    """A resident volume of 256 or more slices that fits whole is cut into about four
+    slabs, backprojected alternately into two device buffers so that each slab's
+    download runs under the next slab's backprojection.  The volume is bit for bit the
+    one of a single-buffered plan (slabs of 64 under a memory budget), in both index
+    orders, and progress reaches 1 in order."""
+    import torch
+    import ct_amd
+    from ct_amd.phantom import circular_angles, random_ellipsoids
+    W = H = 256
+    P, n, nz = 40, 64, 300
+    g = ct_amd.geometry_grid(W, H, 700.0, 0.0, n, n, nz)
+    ctx = ct_amd.Context(0)
+    ctx.set_geometry(g)
+    ctx.set_projections(circular_angles(P))
+    raw_d = torch.empty((P, H, W), dtype=torch.int16, device=gpu)
+    ctx.phantom_images(random_ellipsoids(6, 7, 20.0, mu_range=(0.005, 0.02)), 0, P, 0.0, 65535.0, 7, raw_d, None)
+    torch.cuda.synchronize()
+    raw = raw_d.cpu().numpy()
+
+    def recon(order):
+        out = np.empty(n * n * nz, np.float32)
+        seen = []
+        ctx.reconstruct(raw.ctypes.data, "u16", P, 1.0, "cpu", "ramlak", "f32", out.ctypes.data, order,
+                        progress=lambda f, u: seen.append(f))
+        assert seen == sorted(seen) and seen[-1] == 1.0, seen
+        return out, len(seen)
+
+    assert ctx.plan_reconstruct("u16", P, "f32") == (True, 96)
+    zf, nprog = recon("zfastest")
+    assert nprog == 4
+    xf, _ = recon("xfastest")
+    np.testing.assert_array_equal(xf.reshape(nz, n, n), np.transpose(zf.reshape(n, n, nz), (2, 1, 0)))
+    img = W * H
+    work = img * P * (4 + 2 * 2) + P * 8
+    filt = img * 16 * P
+    ctx.set_memory_budget(work + filt + n * n * 4 * 2 * 64 + 100)
+    assert ctx.plan_reconstruct("u16", P, "f32") == (True, 64)
+    single, nprog = recon("zfastest")
+    assert nprog == 5
+    np.testing.assert_array_equal(single, zf)
+    assert float(np.abs(zf).max()) > 0.0
+
+
 def test_reconstruct_memory_plans_slabs_and_recomputation(gpu):
     """ctg_reconstruct under memory budgets: resident entries with several slabs give the
     same volume bit for bit; entries re-derived per slab and batch (non-resident) give it
