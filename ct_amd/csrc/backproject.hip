@@ -502,7 +502,9 @@ BpArgs make_args(const ctg_geometry& g, int nproj, int z0, int nz, int accumulat
 int sample_max_box(const ctg_geometry& g, const float4* hp, int nproj, int z0, int nz, int zpl, int dma16,
                    int pmod, int prem) {
     const int ntx = (g.nx + kTX - 1) / kTX, nty = (g.ny + kTY - 1) / kTY, nzb = (nz + zpl - 1) / zpl;
-    const int pstep = std::max(1, nproj / 48);
+    // about 16 projections: boxes change slowly with the angle, and the capacity keeps a
+    // margin above the largest sampled box
+    const int pstep = std::max(1, nproj / 16);
     int best = 0;
     const int zblocks[3] = {0, nzb / 2, nzb - 1};
     for (int zi = 0; zi < 3; ++zi) {
@@ -552,6 +554,33 @@ int sample_max_box(const ctg_geometry& g, const float4* hp, int nproj, int z0, i
     return best;
 }
 
+// sample_max_box behind a small per-thread cache: callers that backproject the same
+// projections into the same slab again (iterative methods, slab by slab) pay the host
+// sampling once.  The key hashes the geometry, the projection parameters and the slab;
+// a collision can only mis-size the LDS tile, which costs speed, not correctness
+// (boxes beyond the capacity read global memory).
+int cached_max_box(const ctg_geometry& g, const float4* hp, int nproj, int z0, int nz, int zpl, int dma16, int pmod,
+                   int prem) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    };
+    const int keys[7] = {nproj, z0, nz, zpl, dma16, pmod, prem};
+    mix(&g, sizeof(g));
+    mix(keys, sizeof(keys));
+    mix(hp, sizeof(float4) * (size_t)nproj);
+    struct Entry { uint64_t key; int cap; bool used; };
+    static thread_local Entry cache[16] = {};
+    static thread_local int next = 0;
+    for (const Entry& e : cache)
+        if (e.used && e.key == h) return e.cap;
+    const int cap = sample_max_box(g, hp, nproj, z0, nz, zpl, dma16, pmod, prem);
+    cache[next] = {h, cap, true};
+    next = (next + 1) % 16;
+    return cap;
+}
+
 template <int ZPL>
 ctg_status launch_zpl(const BpLaunch& L, BpArgs a, hipStream_t st) {
     const ctg_geometry& g = *L.g;
@@ -577,7 +606,7 @@ ctg_status launch_zpl(const BpLaunch& L, BpArgs a, hipStream_t st) {
             a.pitch_rem = r % m;
         }
     }
-    int cap = L.h_params ? sample_max_box(g, L.h_params, L.nproj, L.z0 - a.zshift, L.nz + a.zshift, ZPL, a.dma16,
+    int cap = L.h_params ? cached_max_box(g, L.h_params, L.nproj, L.z0 - a.zshift, L.nz + a.zshift, ZPL, a.dma16,
                                           a.pitch_mod, a.pitch_rem)
                          : 0;
     cap = cap + cap / 32 + 16;
