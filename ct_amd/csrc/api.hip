@@ -66,6 +66,27 @@ void DevBuf::release() {
     bytes = 0;
 }
 
+ctg_status HostBuf::reserve(size_t n) {
+    if (n <= bytes && ptr) return CTG_OK;
+    release();
+    if (n == 0) return CTG_OK;
+    hipError_t e = hipHostMalloc(&ptr, n, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        bytes = 0;
+        (void)hipGetLastError();
+        return fail(CTG_ERROR_OUT_OF_MEMORY, "pinned host allocation of %zu bytes failed: %s", n, hipGetErrorString(e));
+    }
+    bytes = n;
+    return CTG_OK;
+}
+
+void HostBuf::release() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
 ctg_status fill_u32(void* d, uint32_t value, size_t count, hipStream_t st) {
     if (count == 0) return CTG_OK;
     const unsigned blocks = (unsigned)std::min<size_t>((count + 255) / 256, 8192);
@@ -149,7 +170,7 @@ ctg_status preprocess_batched(ctg_context* ctx, const void* d_raw, ctg_pixel_typ
 // Host copy split over worker threads (a single thread copies pageable memory at a
 // fraction of what the pinned upload that follows can take).
 void parallel_memcpy(void* dst, const void* src, size_t n) {
-    const size_t chunk = (size_t)32 << 20;
+    const size_t chunk = (size_t)4 << 20;  // at least 4 MiB per thread
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     const size_t workers = std::min<size_t>(std::min<size_t>(8, hw), (n + chunk - 1) / chunk);
     if (workers <= 1) {
@@ -328,6 +349,12 @@ ctg_status ctg_destroy(ctg_context* ctx) {
     ctx->scales_tmp.release();
     ctx->filtered.release();
     ctx->raw_stage.release();
+    for (int i = 0; i < 2; ++i) {
+        ctx->slab[i].release();
+        ctx->pin_up[i].release();
+        ctx->pin_dl[i].release();
+    }
+    ctx->slab_conv.release();
     ctx->ell.release();
     delete ctx;
     return CTG_OK;
@@ -567,7 +594,8 @@ ctg_status plan_reconstruct(ctg_context* ctx, ctg_pixel_type pixel, int nproj, c
     CTG_HIP(hipMemGetInfo(&fr, &tot));
     const long long spare = (long long)ctx->spare_mib * 1024ll * 1024ll;
     long long avail = (long long)fr - spare;
-    avail += (long long)ctx->filtered.bytes + (long long)ctx->raw_stage.bytes + (long long)ctx->rows_f32.bytes;
+    avail += (long long)ctx->filtered.bytes + (long long)ctx->raw_stage.bytes + (long long)ctx->rows_f32.bytes +
+             (long long)ctx->slab[0].bytes + (long long)ctx->slab[1].bytes + (long long)ctx->slab_conv.bytes;
     if (ctx->budget_bytes > 0) avail = std::min<long long>(avail, (long long)ctx->budget_bytes);
     plan.batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nproj, ((size_t)256 << 20) / (img * 4)));
     const long long work = (long long)img * plan.batch * (4 + 2 * pb) + (long long)nproj * 8;
@@ -645,6 +673,7 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
     hipStream_t s_up = nullptr, s_pre = nullptr, s_cmp = nullptr, s_dl = nullptr;
     hipEvent_t ev_up[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr}, ev_dl[2] = {nullptr, nullptr},
                ev_slab[2] = {nullptr, nullptr}, ev_bp = nullptr;
+    // pinned staging and device slabs are the context's, kept for the next call
     void* pinned[2] = {nullptr, nullptr};
     void* pinned_dl[2] = {nullptr, nullptr};
     float* d_slab[2] = {nullptr, nullptr};
@@ -660,16 +689,12 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
             if (ev_done[i]) (void)hipEventDestroy(ev_done[i]);
             if (ev_dl[i]) (void)hipEventDestroy(ev_dl[i]);
             if (ev_slab[i]) (void)hipEventDestroy(ev_slab[i]);
-            if (pinned[i]) (void)hipHostFree(pinned[i]);
-            if (pinned_dl[i]) (void)hipHostFree(pinned_dl[i]);
-            if (d_slab[i]) (void)hipFree(d_slab[i]);
         }
         if (ev_bp) (void)hipEventDestroy(ev_bp);
         if (s_up) (void)hipStreamDestroy(s_up);
         if (s_pre) (void)hipStreamDestroy(s_pre);
         if (s_cmp) (void)hipStreamDestroy(s_cmp);
         if (s_dl) (void)hipStreamDestroy(s_dl);
-        if (d_conv) (void)hipFree(d_conv);
     };
 #define PIPE_HIP(call)                                                                                      \
     do {                                                                                                    \
@@ -704,8 +729,10 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
         PIPE_HIP(hipEventCreateWithFlags(&ev_done[i], hipEventDisableTiming));
         PIPE_HIP(hipEventCreateWithFlags(&ev_dl[i], hipEventDisableTiming));
         PIPE_HIP(hipEventCreateWithFlags(&ev_slab[i], hipEventDisableTiming));
-        PIPE_HIP(hipHostMalloc(&pinned[i], stage_bytes, hipHostMallocDefault));
-        PIPE_HIP(hipHostMalloc(&pinned_dl[i], dl_bytes, hipHostMallocDefault));
+        PIPE_TRY(ctx->pin_up[i].reserve(stage_bytes));
+        PIPE_TRY(ctx->pin_dl[i].reserve(dl_bytes));
+        pinned[i] = ctx->pin_up[i].ptr;
+        pinned_dl[i] = ctx->pin_dl[i].ptr;
     }
     PIPE_TRY(ctx->raw_stage.reserve(img * pb * batch * 2));
     // resident: every projection's entries stay in HBM for all slabs; otherwise one
@@ -727,11 +754,16 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
     }
     const bool xcut = part_x < g.nx;
     const size_t slab_floats = xcut ? (size_t)part_x * g.ny * g.nz : col * (size_t)slab_nz;
-    PIPE_HIP(hipMalloc((void**)&d_slab[0], slab_floats * 4));
-    if (dbl) PIPE_HIP(hipMalloc((void**)&d_slab[1], slab_floats * 4));
+    for (int i = 0; i < (dbl ? 2 : 1); ++i) {
+        PIPE_TRY(ctx->slab[i].reserve(slab_floats * 4));
+        d_slab[i] = ctx->slab[i].as<float>();
+    }
     int cur = 0;                     // slab buffer being backprojected
     ctg_geometry gcur = g;           // grid of the slab being backprojected
-    if (order == CTG_X_FASTEST) PIPE_HIP(hipMalloc(&d_conv, col * 4 * (size_t)slab_nz));
+    if (order == CTG_X_FASTEST) {
+        PIPE_TRY(ctx->slab_conv.reserve(col * 4 * (size_t)slab_nz));
+        d_conv = ctx->slab_conv.ptr;
+    }
     const int nb_total = (nproj + batch - 1) / batch;
 
     // Upload batch b through pinned staging and preprocess it on s_pre into `out`

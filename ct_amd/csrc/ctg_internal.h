@@ -63,9 +63,18 @@ struct DevBuf {
     template <class T> T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Grow-only pinned host buffer owned by a context (hipHostMalloc).
+struct HostBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    ctg_status reserve(size_t n);
+    void release();
+};
+
 // Cached filter response for one (W, filter): full-length real even response
-// Htilde(k) (W values), FFT twiddles (W/2 complex), and for non power-of-two W the
-// circular-convolution kernel g (W values); float and double copies.
+// Htilde(k) (W values), twiddles of the transform length M (M/2 complex), and for
+// widths that are not powers of two the M-point transform of the periodically extended
+// filter kernel (M complex); float and double copies.
 struct FilterTables {
     int W = -1;
     int filter = -1;
@@ -94,6 +103,10 @@ struct ctg_context {
     ctg::DevBuf scales_tmp;          // per-projection scales of internally produced layouts
     ctg::DevBuf filtered;            // backprojection-layout projections (pipeline)
     ctg::DevBuf raw_stage;           // raw images on the device (pipeline)
+    ctg::DevBuf slab[2];             // volume slabs (pipeline)
+    ctg::DevBuf slab_conv;           // index-order converted slab (pipeline)
+    ctg::HostBuf pin_up[2];          // pinned staging of raw images (pipeline)
+    ctg::HostBuf pin_dl[2];          // pinned download chunks (pipeline)
     ctg::DevBuf ell;                 // phantom ellipsoids
 };
 
