@@ -138,6 +138,21 @@ def main():
         torch.cuda.synchronize()
         log(f"calibration copy: {n * 4} bytes read and written, twice")
         del src, dst
+    # measured HBM copy rate beside the published 8 TB/s: 1 GiB read and 1 GiB written
+    # per copy, timed with events over 5 copies after one warm-up
+    n = (1 << 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 5 * 2 * n * 4 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    log(f"device copy: {copy_gbs:.0f} GB/s (read + write)")
+    del src, dst
     ctx = ct_amd.Context(0)
     g, raw, angles, base = build_workload(ctx, torch, args)
     P = args.nproj
@@ -324,6 +339,7 @@ def main():
         "bp_gups_f32": updates / bp32 / 1e9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "hbm_copy_measured_gbs": copy_gbs,
         "cpu_baseline": cpu,
     }
     if "f16" in res:
