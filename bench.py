@@ -80,9 +80,22 @@ def cpu_baseline(g, filtered_rows, angles, seconds):
     nz = max(1, min(g.nz, int(seconds / max(t1, 1e-3))))
     t = run(nz)
     ups = g.nx * g.ny * nz * P
-    return {"value": ups / t / 1e9, "unit": "GUPS", "cores": orc.get_threads(), "kind": "port",
+    threads = orc.get_threads()
+    # one thread on a single slice (about a tenth of the multi-threaded sample's time)
+    orc.set_threads(1)
+    t_single = run(1)
+    orc.set_threads(0)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"value": ups / t / 1e9, "unit": "GUPS", "cores": threads, "kind": "port",
             "sample": f"CPU oracle backprojection (reconstructionCore restatement, OpenMP) of {P} projections "
-                      f"into a {g.nx}x{g.ny}x{nz} central z-slab of the same grid, {t:.1f} s wall"}
+                      f"into a {g.nx}x{g.ny}x{nz} central z-slab of the same grid, {t:.1f} s wall",
+            "single_thread_gups": g.nx * g.ny * P / t_single / 1e9,
+            "cpu_model": model, "host_logical_cpus": os.cpu_count()}
 
 
 def main():
