@@ -116,7 +116,10 @@ ctg_status ctg_geometry_grid(uint32_t det_width, uint32_t det_height, float fcd_
 
 /* ---------------------------------------------------------------- context */
 
-/* Binds a device (cudaSetDevice in cudaReconstructionCore, Ct/CtVolume.cpp:1060). */
+/* Binds a device (cudaSetDevice in cudaReconstructionCore, Ct/CtVolume.cpp:1060).
+ * A context keeps the working buffers of its calls (filter tables, staged and
+ * filtered projections, volume slabs, pinned host staging) for the next call;
+ * ctg_destroy releases them. */
 ctg_status ctg_create(int32_t device, ctg_context** out);
 ctg_status ctg_destroy(ctg_context* ctx);
 ctg_status ctg_set_geometry(ctg_context* ctx, const ctg_geometry* geometry);
