@@ -308,8 +308,10 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
                 auto process = [&](const Slot& s) {
                     float ta[G], tb[G];
                     int ix[G];
+                    // world z of the chunks: exact half-integers, so chunk k's is chunk 0's + 64 k
+                    const float zw0 = zworld(s.zc);
 #pragma unroll
-                    for (int k = 0; k < G; ++k) ix[k] = tap(zworld(s.zc + 64 * k), s.x[k], s.it.U, s.it.wc, ta[k], tb[k]);
+                    for (int k = 0; k < G; ++k) ix[k] = tap(zw0 + (float)(64 * k), s.x[k], s.it.U, s.it.wc, ta[k], tb[k]);
                     // loads before stores: the 2G rows of a pass are distinct
                     auto rmw = [&]() {
                         float r0[G], r1[G];
