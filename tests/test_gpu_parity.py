@@ -323,6 +323,46 @@ def test_forward_projector_matches_fp64_oracle(gpu, oracle):
     assert err < 1e-5
 
 
+def test_forward_projector_strong_magnification(gpu, oracle):
+    """A cylinder of 0.7 FCD (beyond what the CtVolume rules give, below the 0.9 FCD the
+    projector accepts) puts voxels at magnifications U up to 3.3: their residue split
+    takes one pass (U >= 2), two (1 <= U < 2) or four (U < 1) within one projection."""
+    import ctypes
+    import torch
+    import ct_amd
+    from ct_amd import _ctg
+    W, H, P, n, nz = 64, 48, 8, 96, 40
+    base = ct_amd.geometry_grid(W, H, 65.0, 0.0, n, n, nz)
+    g = _ctg.Geometry()
+    ctypes.memmove(ctypes.byref(g), ctypes.byref(base), ctypes.sizeof(g))
+    g.radius = 0.7 * g.fcd
+    ctx = ct_amd.Context(0)
+    ctx.set_geometry(g)
+    angles = np.linspace(0.0, 360.0, P, endpoint=False)
+    heights = np.linspace(-1.0, 1.5, P)
+    ctx.set_projections(angles, heights)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((n, n, nz)).astype(np.float32)
+    y = rng.standard_normal((P, H, W)).astype(np.float32)
+    xd, yd = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    Ax = torch.empty((P, H, W), dtype=torch.float32, device="cuda")
+    ctx.forward_project(xd, 0, nz, 0, P, Ax)
+    yl = torch.empty(ctx.filtered_bytes("f32", P), dtype=torch.uint8, device="cuda")
+    ctx.layout_projections(yd, P, "f32", yl, None)
+    Aty = torch.empty_like(xd)
+    ctx.backproject(yl, "f32", None, 0, P, Aty, 0, nz)
+    torch.cuda.synchronize()
+    lhs = float(np.sum(Ax.cpu().numpy().astype(np.float64) * y))
+    rhs = float(np.sum(x.astype(np.float64) * Aty.cpu().numpy()))
+    rel = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    ref = oracle.forward_project_f64(_oracle_geom(g), x.astype(np.float64), angles, heights)
+    err = rel_rmse(Ax.cpu().numpy(), ref)
+    print(f"strong magnification: dot {rel:.2e}, vs FP64 oracle {err:.2e}")
+    assert float(np.abs(ref).max()) > 0.0
+    assert rel < 1e-5
+    assert err < 1e-5
+
+
 def test_forward_projector_rejects_geometry_outside_its_bounds(gpu):
     """The projector's row accumulators and residue passes rely on the reconstruction
     cylinder lying well inside the source circle and on the v window lying inside the
