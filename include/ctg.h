@@ -148,9 +148,10 @@ size_t ctg_filtered_bytes(const ctg_geometry* geometry, ctg_store store, int32_t
  * weight, ramp-family row filter (real DFT of length W without padding, bins
  * 0..W/2 times the window, unnormalised inverse), then conversion into the
  * backprojection layout in the chosen store with per-projection power-of-two
- * scales written to d_scales (1 for F32).  Any width up to the LDS bound is
- * filtered (powers of two up to 8192 in FP32; other widths up to 4096 through a
- * padded power-of-two transform); wider returns CTG_ERROR_UNSUPPORTED.
+ * scales written to d_scales (1 for F32).  The filter runs a transform of length
+ * M = W for powers of two, else the power of two >= 2W - 1 (padded); M up to 16384
+ * fits the LDS row buffer (8192 for the FP64 yardstick, ctg_fdk_f64), and wider
+ * rows return CTG_ERROR_UNSUPPORTED.
  * Replaces cudaPreprocessImage (Ct/CtVolume.cpp:887-921): convertTo/log/multiply,
  * applyFeldkampWeightFiltering (Ct/cuda_ct.cu:119-131), FftFilter::applyForward/
  * applyInverse (Ct/CtVolume.cpp:75-101) and applyFrequencyFiltering (Ct/cuda_ct.cu:87-100). */
