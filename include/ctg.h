@@ -223,8 +223,10 @@ ctg_status ctg_convert_volume(ctg_context* ctx, const float* d_volume, int32_t n
  * a host volume in the requested index order (nx*ny*nz floats).  Plans z-slabs
  * from free VRAM minus the spare memory, keeps filtered projections resident
  * when they fit and re-filters per slab otherwise, overlaps the upload of raw
- * images with preprocessing on separate streams.  Replaces
- * launchCudaThreads/cudaReconstructionCore (Ct/CtVolume.cpp:1056-1397). */
+ * images with preprocessing and backprojection, and each slab's download with the
+ * next slab's backprojection (see ctg_plan_reconstruct), on separate streams.
+ * Progress is reported per slab.  Replaces launchCudaThreads/cudaReconstructionCore
+ * (Ct/CtVolume.cpp:1056-1397). */
 ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type pixel,
                            int32_t nproj, float base_intensity, ctg_log_form log_form,
                            ctg_filter filter, ctg_store store, float* h_volume,
