@@ -53,7 +53,8 @@ def test_random_geometries_fdk_and_adjoint(gpu, oracle, seed):
     """Seeded random scans: detector sizes (odd, even, power of two or not), FCD from
     1.1 to 6 detector widths, rotation-axis offsets, per-projection heights, uneven
     angles over less or more than a turn, all three windows and both log forms.  FDK
-    against the oracle at the FP32 bar, and the projector pair's dot test."""
+    against the oracle at the FP32 bar, FP16-stored projections against the FP64 oracle
+    at 1e-3, and the projector pair's dot test."""
     import torch
     rng = np.random.default_rng(seed)
     W = int(rng.choice([33, 48, 70, 96, 128]))
@@ -71,6 +72,10 @@ def test_random_geometries_fdk_and_adjoint(gpu, oracle, seed):
     got, _, _ = _gpu_fdk(ctx, g, scan, filt=filt, log_form=log_form)
     ref = oracle.fdk_f32(_oracle_geom(g), scan["raw"], "u16", scan["base"], filt, angles, heights, log_form)
     err = rel_rmse(got, ref)
+    # FP16-stored projections against the FP64 oracle (CPU log form, as fdk_f64)
+    f16, _, _ = _gpu_fdk(ctx, g, scan, store="f16", filt=filt, log_form="cpu")
+    ref64 = oracle.fdk_f64(_oracle_geom(g), scan["raw"], "u16", scan["base"], filt, angles, heights)
+    err16 = rel_rmse(f16, ref64)
     # adjoint pair on the same geometry
     x = rng.standard_normal((g.nx, g.ny, g.nz)).astype(np.float32)
     y = rng.standard_normal((P, H, W)).astype(np.float32)
@@ -86,8 +91,9 @@ def test_random_geometries_fdk_and_adjoint(gpu, oracle, seed):
     rhs = float(np.sum(x.astype(np.float64) * Aty.cpu().numpy()))
     dot = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
     print(f"seed {seed}: W={W} H={H} P={P} FCD={fcd:.1f} uoff={u_offset:.2f} {filt}/{log_form}: "
-          f"FDK vs oracle {err:.2e}, dot {dot:.2e}")
+          f"FDK vs oracle {err:.2e}, FP16 store vs FP64 {err16:.2e}, dot {dot:.2e}")
     assert err < FDK_TOL
+    assert err16 < 1e-3
     assert dot < 1e-5
 
 
