@@ -143,6 +143,19 @@ ctg_status check_slab(ctg_context* ctx, int z0, int nz) {
     return CTG_OK;
 }
 
+bool bad_pixel(ctg_pixel_type v) { return (int)v < 0 || (int)v > 2; }
+bool bad_filter(ctg_filter v) { return (int)v < 0 || (int)v > 2; }
+bool bad_store(ctg_store v) { return (int)v < 0 || (int)v > 2; }
+bool bad_log_form(ctg_log_form v) { return (int)v < 0 || (int)v > 1; }
+
+// FP16/BF16 entries are stored divided by a per-projection power of two: without the
+// scale array the backprojection would silently use 1
+ctg_status check_scales(ctg_store store, const float* d_scales, int nproj) {
+    if (nproj > 0 && store != CTG_STORE_F32 && !d_scales)
+        return fail(CTG_ERROR_INVALID_ARGUMENT, "FP16/BF16 stores need a scale array");
+    return CTG_OK;
+}
+
 // Preprocess nproj raw images into the BP layout in batches that bound the FP32 work buffer.
 ctg_status preprocess_batched(ctg_context* ctx, const void* d_raw, ctg_pixel_type pixel, int nproj, float base,
                               ctg_log_form form, ctg_filter filter, ctg_store store, void* d_filtered,
@@ -430,8 +443,9 @@ ctg_status ctg_preprocess(ctg_context* ctx, const void* d_raw, ctg_pixel_type pi
         return fail(CTG_ERROR_INVALID_ARGUMENT, "bad buffers");
     if (base_intensity == 0.0f)  // Ct/CtVolume.cpp:177-181
         return fail(CTG_ERROR_INVALID_ARGUMENT, "the base intensity must not be equal to zero");
-    if ((int)pixel < 0 || (int)pixel > 2 || (int)filter < 0 || (int)filter > 2 || (int)store < 0 || (int)store > 2)
+    if (bad_pixel(pixel) || bad_filter(filter) || bad_store(store) || bad_log_form(log_form))
         return fail(CTG_ERROR_INVALID_ARGUMENT, "bad enum value");
+    CTG_TRY(check_scales(store, d_scales, nproj));
     if (nproj == 0) return CTG_OK;
     return preprocess_batched(ctx, d_raw, pixel, nproj, base_intensity, log_form, filter, store, d_filtered,
                               d_scales, (hipStream_t)stream);
@@ -440,7 +454,8 @@ ctg_status ctg_preprocess(ctg_context* ctx, const void* d_raw, ctg_pixel_type pi
 ctg_status ctg_feldkamp_weight(ctg_context* ctx, float* d_images, int32_t nproj, void* stream) {
     clear_error();
     CTG_TRY(check_ctx(ctx));
-    if (nproj <= 0) return CTG_OK;
+    if (nproj < 0 || (nproj > 0 && !d_images)) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad buffers");
+    if (nproj == 0) return CTG_OK;
     return launch_preprocess_rows(ctx, d_images, CTG_PIXEL_F32, nproj, 1.0f, CTG_LOG_CPU_FORM, CTG_FILTER_RAMLAK,
                                   d_images, nullptr, false, true, false, (hipStream_t)stream);
 }
@@ -449,8 +464,9 @@ ctg_status ctg_filter_rows(ctg_context* ctx, const float* d_in, float* d_out, in
                            void* stream) {
     clear_error();
     CTG_TRY(check_ctx(ctx));
-    if (rows <= 0) return CTG_OK;
-    if (!d_in || !d_out) return fail(CTG_ERROR_INVALID_ARGUMENT, "null buffers");
+    if (bad_filter(filter)) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad filter");
+    if (rows < 0 || (rows > 0 && (!d_in || !d_out))) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad buffers");
+    if (rows == 0) return CTG_OK;
     return launch_filter_rows_only(ctx, d_in, d_out, rows, filter, (hipStream_t)stream);
 }
 
@@ -458,7 +474,10 @@ ctg_status ctg_layout_projections(ctg_context* ctx, const float* d_images, int32
                                   void* d_filtered, float* d_scales, void* stream) {
     clear_error();
     CTG_TRY(check_ctx(ctx));
-    if (nproj <= 0) return CTG_OK;
+    if (bad_store(store)) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad store");
+    if (nproj < 0 || (nproj > 0 && (!d_images || !d_filtered))) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad buffers");
+    CTG_TRY(check_scales(store, d_scales, nproj));
+    if (nproj == 0) return CTG_OK;
     hipStream_t st = (hipStream_t)stream;
     const size_t img = (size_t)ctx->geom.det_width * ctx->geom.det_height;
     CTG_TRY(ctx->maxabs.reserve(sizeof(unsigned) * nproj));
@@ -479,9 +498,14 @@ ctg_status ctg_backproject(ctg_context* ctx, const void* d_filtered, ctg_store s
     CTG_TRY(check_ctx(ctx));
     CTG_TRY(check_range(ctx, p0, nproj));
     CTG_TRY(check_slab(ctx, z0, nz));
-    if ((int)store < 0 || (int)store > 2) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad store");
+    if (bad_store(store)) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad store");
     if (!d_volume || (nproj > 0 && !d_filtered)) return fail(CTG_ERROR_INVALID_ARGUMENT, "null buffers");
+    CTG_TRY(check_scales(store, d_scales, nproj));
     if (ctx->cancel.load()) return fail(CTG_ERROR_INTERRUPTED, "interrupted");
+    if (nproj == 0) {  // no projection to stage: an overwrite leaves a zero slab
+        if (accumulate) return CTG_OK;
+        return fill_u32(d_volume, 0u, (size_t)ctx->geom.nx * ctx->geom.ny * nz, (hipStream_t)stream);
+    }
     BpLaunch L{};
     L.g = &ctx->geom;
     L.d_params = ctx->proj_dev.as<float4>() + p0;
@@ -504,6 +528,7 @@ ctg_status ctg_forward_project(ctg_context* ctx, const float* d_volume, int32_t 
     CTG_TRY(check_range(ctx, p0, nproj));
     CTG_TRY(check_slab(ctx, z0, nz));
     if (!d_volume || (nproj > 0 && !d_proj)) return fail(CTG_ERROR_INVALID_ARGUMENT, "null buffers");
+    if (nproj == 0) return CTG_OK;
     hipStream_t st = (hipStream_t)stream;
     const size_t n = (size_t)nproj * ctx->geom.det_width * ctx->geom.det_height;
     if (!accumulate) CTG_TRY(fill_u32(d_proj, 0u, n, st));
@@ -516,6 +541,8 @@ ctg_status ctg_fdk_backward(ctg_context* ctx, const float* d_grad_volume, int32_
     CTG_TRY(check_ctx(ctx));
     CTG_TRY(check_range(ctx, p0, nproj));
     CTG_TRY(check_slab(ctx, z0, nz));
+    if (bad_filter(filter)) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad filter");
+    if (nproj > 0 && (!d_grad_volume || !d_grad_lineint)) return fail(CTG_ERROR_INVALID_ARGUMENT, "null buffers");
     if (nproj == 0) return CTG_OK;
     hipStream_t st = (hipStream_t)stream;
     const ctg_geometry& g = ctx->geom;
@@ -535,12 +562,15 @@ ctg_status ctg_fdk_f64(ctg_context* ctx, const void* d_raw, ctg_pixel_type pixel
     CTG_TRY(check_range(ctx, 0, nproj));
     CTG_TRY(check_slab(ctx, z0, nz));
     if (base_intensity == 0.0f) return fail(CTG_ERROR_INVALID_ARGUMENT, "the base intensity must not be equal to zero");
+    if (bad_pixel(pixel) || bad_filter(filter)) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad enum value");
+    if ((nz > 0 && !d_volume) || (nproj > 0 && !d_raw)) return fail(CTG_ERROR_INVALID_ARGUMENT, "null buffers");
     hipStream_t st = (hipStream_t)stream;
     const ctg_geometry& g = ctx->geom;
     const size_t img = (size_t)g.det_width * g.det_height;
     CTG_TRY(ctx->rows_f64.reserve(img * 8 * std::max(nproj, 1)));
-    CTG_TRY(launch_preprocess_rows_f64(ctx, d_raw, pixel, nproj, base_intensity, filter, ctx->rows_f64.as<double>(),
-                                       st));
+    if (nproj > 0)
+        CTG_TRY(launch_preprocess_rows_f64(ctx, d_raw, pixel, nproj, base_intensity, filter,
+                                           ctx->rows_f64.as<double>(), st));
     return launch_backproject_f64(g, ctx->proj_dev64.as<double4>(), ctx->rows_f64.as<double>(), nproj, d_volume, z0,
                                   nz, st);
 }
@@ -550,6 +580,10 @@ ctg_status ctg_convert_volume(ctg_context* ctx, const float* d_volume, int32_t n
     clear_error();
     CTG_TRY(check_ctx(ctx));
     if (!d_volume || !d_out || nz < 0) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad buffers");
+    if (nz > ctx->geom.nz) return fail(CTG_ERROR_INVALID_ARGUMENT, "slab of %d slices exceeds the grid's %d", nz, ctx->geom.nz);
+    if ((int)order < 0 || (int)order > 1 || (int)byte_order < 0 || (int)byte_order > 1)
+        return fail(CTG_ERROR_INVALID_ARGUMENT, "bad enum value");
+    if (nz == 0) return CTG_OK;
     return launch_convert_volume(ctx->geom, d_volume, nz, order, byte_order, d_out, (hipStream_t)stream);
 }
 
@@ -651,6 +685,8 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
     CTG_TRY(check_range(ctx, 0, nproj));
     if (!h_raw || !h_volume || nproj <= 0) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad buffers");
     if (base_intensity == 0.0f) return fail(CTG_ERROR_INVALID_ARGUMENT, "the base intensity must not be equal to zero");
+    if (bad_filter(filter) || bad_log_form(log_form) || (int)order < 0 || (int)order > 1)
+        return fail(CTG_ERROR_INVALID_ARGUMENT, "bad filter, log form or index order");
     const ctg_geometry& g = ctx->geom;
     const size_t img = (size_t)g.det_width * g.det_height;
     const size_t pb = pixel_bytes(pixel);

@@ -148,7 +148,8 @@ size_t ctg_filtered_bytes(const ctg_geometry* geometry, ctg_store store, int32_t
  * weight, ramp-family row filter (real DFT of length W without padding, bins
  * 0..W/2 times the window, unnormalised inverse), then conversion into the
  * backprojection layout in the chosen store with per-projection power-of-two
- * scales written to d_scales (1 for F32).  The filter runs a transform of length
+ * scales written to d_scales (nproj floats, 1 for F32; may be null for F32 only).
+ * The filter runs a transform of length
  * M = W for powers of two, else the power of two >= 2W - 1 (padded); M up to 16384
  * fits the LDS row buffer (8192 for the FP64 yardstick, ctg_fdk_f64), and wider
  * rows return CTG_ERROR_UNSUPPORTED.
@@ -171,7 +172,8 @@ ctg_status ctg_filter_rows(ctg_context* ctx, const float* d_in, float* d_out, in
                            ctg_filter filter, void* stream);
 
 /* Row-major FP32 detector images -> backprojection layout in a store type, with
- * per-projection power-of-two scales (no counterpart: the reference kept FP32). */
+ * per-projection power-of-two scales in d_scales (may be null for F32 only; no
+ * counterpart: the reference kept FP32). */
 ctg_status ctg_layout_projections(ctg_context* ctx, const float* d_images, int32_t nproj,
                                   ctg_store store, void* d_filtered, float* d_scales,
                                   void* stream);
@@ -179,8 +181,10 @@ ctg_status ctg_layout_projections(ctg_context* ctx, const float* d_images, int32
 /* Voxel-driven backprojection of projections [p0, p0+nproj) (geometry indices
  * from ctg_set_projections; d_filtered holds them in order starting with p0)
  * into the slab d_volume = z-range [z0, z0+nz).  accumulate = 0 overwrites the
- * slab, 1 adds to it.  Replaces startReconstruction/reconstructionKernel
- * (Ct/cuda_ct.cu:201-352) with one launch for many projections. */
+ * slab, 1 adds to it (nproj = 0 then zeroes, or leaves, the slab).  d_scales: the
+ * scales ctg_preprocess wrote, required for F16/BF16 (null means 1 for F32).
+ * Replaces startReconstruction/reconstructionKernel (Ct/cuda_ct.cu:201-352) with one
+ * launch for many projections. */
 ctg_status ctg_backproject(ctg_context* ctx, const void* d_filtered, ctg_store store,
                            const float* d_scales, int32_t p0, int32_t nproj,
                            float* d_volume, int32_t z0, int32_t nz, int32_t accumulate,
