@@ -15,6 +15,10 @@ refine768      sparse view: 120 projections of 1024x768 (FCD 5000 px), 768^3 vol
                filtered projections, then 10 steepest-descent steps (exact line search) on
                0.5 ||A x - p||^2 with the adjoint pair, in z-slabs; RMSE against the
                voxelised analytic phantom and against the plain FP32 FDK, ms per step.
+rowfilter1024  the ramp-family row filter over 720 x 1024 rows of 1024 (Shepp-Logan):
+               ctg_filter_rows (the register FFT) against the same circular filter as a
+               dense circulant GEMM on the matrix cores (torch.matmul with FP32 inputs,
+               and with BF16 inputs for scale), with the GEMMs' error against the FFT.
 
 Each mode prints one JSON line.  Synthetic data are generated on the GPU.
 """
@@ -240,4 +244,66 @@ def refine768(args):
     print(json.dumps(out), flush=True)
 
 
-WORKLOADS = {"precision2048": precision2048, "gradient512": gradient512, "refine768": refine768}
+def rowfilter1024(args):
+    import torch
+    import ct_amd
+
+    dev = torch.device("cuda:0")
+    W, P = args.det_w or 1024, args.nproj or 720
+    H = args.det_h or W
+    rows = P * H
+    filt = "shepplogan"
+    g = ct_amd.geometry_grid(W, H, 2500.0, 0.0, 16, 16, 16)  # the row filter reads only the width
+    ctx = ct_amd.Context(0)
+    ctx.set_geometry(g)
+    stream = torch.cuda.current_stream()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(7)
+    x = torch.randn((rows, W), generator=gen, device=dev)
+    y_fft = torch.empty_like(x)
+    # The filter is a circular convolution, y[j] = sum_i x[i] h[(j - i) mod W], with h the
+    # library's own response to a unit impulse: y = x C with C[i][j] = h[(j - i) mod W].
+    delta = torch.zeros((1, W), device=dev)
+    delta[0, 0] = 1.0
+    h = torch.empty_like(delta)
+    ctx.filter_rows(delta, h, 1, filt, stream.cuda_stream)
+    ar = torch.arange(W, device=dev)
+    C = h[0][(ar[None, :] - ar[:, None]) % W].contiguous()
+    torch.backends.cuda.matmul.allow_tf32 = False
+    xb, Cb = x.bfloat16(), C.bfloat16()
+
+    def timed(fn):
+        fn()
+        _sync(torch)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            r = fn()
+        e1.record(stream)
+        _sync(torch)
+        return e0.elapsed_time(e1) / args.steps, r
+
+    t_fft, _ = timed(lambda: ctx.filter_rows(x, y_fft, rows, filt, stream.cuda_stream))
+    t_f32, y32 = timed(lambda: torch.matmul(x, C))
+    t_bf16, yb = timed(lambda: torch.matmul(xb, Cb))
+
+    def rel(a):
+        d = a.double() - y_fft.double()
+        return float(torch.sqrt(torch.sum(d * d) / torch.sum(y_fft.double() ** 2)))
+
+    flops = 2.0 * rows * W * W
+    out = {"workload": f"row filter: {P} x {H} rows of {W}, {filt}", "rows": rows,
+           "fft_ms": t_fft, "fft_gbs": rows * W * 8 / t_fft / 1e6,
+           "gemm_f32_ms": t_f32, "gemm_f32_tflops": flops / t_f32 / 1e9, "gemm_f32_peak_tflops": 157.3,
+           "gemm_f32_rel_rmse_vs_fft": rel(y32),
+           "gemm_bf16_ms": t_bf16, "gemm_bf16_tflops": flops / t_bf16 / 1e9, "gemm_bf16_peak_tflops": 2500.0,
+           "gemm_bf16_rel_rmse_vs_fft": rel(yb)}
+    log(f"FFT {t_fft:.2f} ms; FP32 GEMM {t_f32:.2f} ms ({out['gemm_f32_tflops']:.0f} TFLOP/s, "
+        f"rel RMSE {out['gemm_f32_rel_rmse_vs_fft']:.1e}); BF16 GEMM {t_bf16:.2f} ms "
+        f"({out['gemm_bf16_tflops']:.0f} TFLOP/s, rel RMSE {out['gemm_bf16_rel_rmse_vs_fft']:.1e})")
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+WORKLOADS = {"precision2048": precision2048, "gradient512": gradient512, "refine768": refine768,
+             "rowfilter1024": rowfilter1024}
