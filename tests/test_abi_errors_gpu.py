@@ -133,3 +133,22 @@ def test_empty_inputs_are_no_ops(setup):
     assert lib.ctg_convert_volume(h, _p(b["vol"]), 0, 0, 1, _p(b["out"]), None) == 0
     torch.cuda.synchronize()
     assert bool((proj == 7.0).all())
+
+
+def test_row_filter_wider_than_the_lds_buffer_is_unsupported(gpu):
+    """W = 8193 would need a padded transform of 32768 points (256 KB of LDS): the call
+    returns CTG_ERROR_UNSUPPORTED before any launch.  8191 and 8192, the widest rows that
+    fit, are checked against the oracle with the other row-filter widths."""
+    import torch
+    import ct_amd
+    from ct_amd import _ctg
+    W, H = 8193, 2
+    g, _ = ct_amd.geometry_ctvolume(W, H, 3.0 * W, 0.0, None)
+    with ct_amd.Context(0) as ctx:
+        ctx.set_geometry(g)
+        ctx.set_projections([0.0])
+        raw = torch.full((1, H, W), 30000, dtype=torch.uint16, device=gpu)
+        filtered = torch.empty(ctx.filtered_bytes("f32", 1), dtype=torch.uint8, device=gpu)
+        st = ctx.lib.ctg_preprocess(ctx.handle, _p(raw), 1, 1, 0.9, 0, 0, 0, _p(filtered), None, None)
+        assert st == _ctg.ERROR_UNSUPPORTED, (st, _ctg.last_error())
+        assert "LDS" in _ctg.last_error()

@@ -171,12 +171,14 @@ def test_register_fft_widths_match_oracle(gpu, oracle, W, H, P, pixel):
     assert rel_rmse(out.cpu().numpy(), ref) < 2e-6
 
 
-@pytest.mark.parametrize("W,H,P", [(90, 5, 2), (1000, 3, 2), (1944, 3, 2), (3000, 2, 2)])
+@pytest.mark.parametrize("W,H,P", [(90, 5, 2), (1000, 3, 2), (1944, 3, 2), (3000, 2, 2), (8191, 2, 2),
+                                   (8192, 2, 2)])
 def test_padded_row_filter_widths_match_oracle(gpu, oracle, W, H, P):
     """Widths that are not powers of two filter through a power-of-two transform of
     length M >= 2W - 1 (the circular convolution as the middle of a linear one); 3000
-    needs M = 8192.  Checked against the oracle for all three windows, and the filter
-    alone against the oracle's FP64 rows."""
+    needs M = 8192 and 8191, the widest such row, M = 16384 (128 KB of LDS).  8192 is the
+    widest power of two, unpadded in the LDS kernel.  Checked against the oracle for all
+    three windows, and the filter alone against the oracle's FP64 rows."""
     import torch
     import ct_amd
     rng = np.random.default_rng(W)
