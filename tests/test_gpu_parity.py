@@ -374,6 +374,55 @@ def test_forward_projector_matches_fp64_oracle(gpu, oracle):
     assert err < 1e-5
 
 
+def test_forward_projector_tallest_detector(gpu, oracle):
+    """4096 detector rows, the most the projector holds in LDS (eight row copies of
+    4100 floats per workgroup), under a 4000-slice column with per-projection heights.
+
+    Detector rows are FP32, as in the reference.  A row index near 4000 resolves the
+    row fraction only to 2^-12, and one ulp of the magnification moves a voxel at
+    |v| ~ 2000 by about 2e-4 rows.  The projector splats each voxel's value with those
+    weights, so even a smooth volume keeps that error: measured 1.9e-5 against the FP64
+    oracle projector (bound 5e-5; an index or weight mistake shows at 1e-2 or more), and
+    1.6e-4 for white noise (bound 4e-4).  The dot test against the backprojection, which
+    uses the same FP32 weights, is unaffected."""
+    import torch
+    import ct_amd
+    W, H, P, nz = 24, 4096, 3, 4000
+    g = ct_amd.geometry_grid(W, H, 300.0, 0.0, 12, 12, nz)
+    angles, heights = np.array([10.0, 130.0, 250.0]), np.array([0.0, 3.5, -7.25])
+    ctx = ct_amd.Context(0)
+    ctx.set_geometry(g)
+    ctx.set_projections(angles, heights)
+    og = _oracle_geom(g)
+    i, j, k = np.meshgrid(np.arange(g.nx), np.arange(g.ny), np.arange(nz), indexing="ij")
+    smooth = (1.0 + 0.5 * np.cos(2 * np.pi * k / 517.0) + 0.2 * np.sin(2 * np.pi * (i + 2 * j) / 11.0))
+    rng = np.random.default_rng(12)
+    noise = rng.standard_normal((g.nx, g.ny, nz))
+    y = rng.standard_normal((P, H, W)).astype(np.float32)
+    Ax = torch.empty((P, H, W), dtype=torch.float32, device="cuda")
+    errs = {}
+    for name, x in (("smooth", smooth), ("noise", noise)):
+        xd = torch.from_numpy(x.astype(np.float32)).cuda()
+        ctx.forward_project(xd, 0, nz, 0, P, Ax)
+        torch.cuda.synchronize()
+        ref = oracle.forward_project_f64(og, x.astype(np.float32).astype(np.float64), angles, heights)
+        errs[name] = rel_rmse(Ax.cpu().numpy(), ref)
+    yd = torch.from_numpy(y).cuda()
+    yl = torch.empty(ctx.filtered_bytes("f32", P), dtype=torch.uint8, device="cuda")
+    ctx.layout_projections(yd, P, "f32", yl, None)
+    Aty = torch.empty_like(xd)
+    ctx.backproject(yl, "f32", None, 0, P, Aty, 0, nz)
+    torch.cuda.synchronize()
+    lhs = float(torch.sum(Ax.double() * yd.double()))
+    rhs = float(torch.sum(xd.double() * Aty.double()))
+    dot = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    print(f"H={H}: projector vs FP64 oracle: smooth {errs['smooth']:.2e}, white noise {errs['noise']:.2e}; "
+          f"dot {dot:.2e}")
+    assert errs["smooth"] < 5e-5
+    assert errs["noise"] < 4e-4
+    assert dot < 1e-5
+
+
 def test_forward_projector_strong_magnification(gpu, oracle):
     """A cylinder of 0.7 FCD (beyond what the CtVolume rules give, below the 0.9 FCD the
     projector accepts) puts voxels at magnifications U up to 3.3: their residue split
