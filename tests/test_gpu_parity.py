@@ -128,16 +128,22 @@ def test_preprocess_matches_oracle(gpu, oracle):
 
 
 @pytest.mark.parametrize("W,H,P,pixel", [(256, 7, 3, "u16"), (512, 5, 3, "u8"), (1024, 9, 2, "u16"),
-                                         (2048, 3, 3, "u16")])
+                                         (2048, 3, 3, "u16"), (256, 4, 2, "f32")])
 def test_register_fft_widths_match_oracle(gpu, oracle, W, H, P, pixel):
     """Widths 256..2048 take the register-resident row filter.  An odd row count puts a
-    row pair across two images and leaves the last pair without its second row."""
+    row pair across two images and leaves the last pair without its second row.  The
+    three image depths the loader accepts (8U, 16U, 32F; Ct/CtVolume.cpp:247) are
+    covered; 32F images are taken as they are, without the integer full-scale division."""
     import torch
     import ct_amd
     rng = np.random.default_rng(W)
-    hi = 255 if pixel == "u8" else 65535
-    dt = np.uint8 if pixel == "u8" else np.uint16
-    raw = rng.integers(hi // 20, hi, size=(P, H, W)).astype(dt)
+    if pixel == "f32":
+        hi = 1.0
+        raw = rng.uniform(0.05, 1.0, size=(P, H, W)).astype(np.float32)
+    else:
+        hi = 255 if pixel == "u8" else 65535
+        dt = np.uint8 if pixel == "u8" else np.uint16
+        raw = rng.integers(hi // 20, hi, size=(P, H, W)).astype(dt)
     g, _ = ct_amd.geometry_ctvolume(W, H, 3.0 * W, 0.0, None)
     ctx = ct_amd.Context(0)
     ctx.set_geometry(g)
