@@ -430,7 +430,7 @@ ctg_status ctg_set_memory_budget(ctg_context* ctx, uint64_t bytes) {
 }
 
 size_t ctg_filtered_bytes(const ctg_geometry* g, ctg_store store, int32_t nproj) {
-    if (!g || nproj < 0) return 0;
+    if (!g || nproj < 0 || bad_store(store)) return 0;
     return (size_t)g->det_width * g->det_height * store_bytes(store) * (size_t)nproj;
 }
 

@@ -141,7 +141,8 @@ ctg_status ctg_set_memory_budget(ctg_context* ctx, uint64_t bytes);
 
 /* ----------------------------------------------------- stream-ordered ops */
 
-/* Bytes of the backprojection layout for nproj projections in a store type. */
+/* Bytes of the backprojection layout for nproj projections in a store type (0 for a
+ * null geometry, a negative count or an unknown store). */
 size_t ctg_filtered_bytes(const ctg_geometry* geometry, ctg_store store, int32_t nproj);
 
 /* Fused preprocessing of nproj raw images: intensity -> line integral, Feldkamp

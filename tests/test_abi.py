@@ -78,6 +78,9 @@ def test_filtered_bytes(ctg_lib):
     assert f(ctypes.byref(g), _ctg.STORE["f32"], 720) == 1024 * 1024 * 16 * 720
     assert f(ctypes.byref(g), _ctg.STORE["f16"], 3) == 1024 * 1024 * 8 * 3
     assert f(ctypes.byref(g), _ctg.STORE["bf16"], 0) == 0
+    assert f(ctypes.byref(g), 3, 4) == 0 and f(ctypes.byref(g), -1, 4) == 0
+    assert f(ctypes.byref(g), _ctg.STORE["f32"], -1) == 0
+    assert f(None, _ctg.STORE["f32"], 4) == 0
 
 
 def test_no_device_is_reported_not_faked():
