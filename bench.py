@@ -124,7 +124,7 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(FDK shards over z-slabs with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "(FDK would shard over z-slabs with no exchange step). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     if args.workload != "fdk1024":
