@@ -123,7 +123,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: ct_amd has no multi-GPU path "
               "(FDK would shard over z-slabs with no exchange step). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 

@@ -22,7 +22,7 @@ void print_help() {
                  "\t-b \t\tOptional. Run with background priority. Long: --background.\n"
                  "\t-f [option] \tOptional. Sets the preprocessing filter. Options are\n"
                  "\t\t\t'ramlak', 'shepplogan' and 'hann'. Long: --filter.\n\t\t\tDefault: ramlak\n"
-                 "\t-n \t\tOptional. Disables GPU reconstruction (not available in this build). Long: --nocuda.\n"
+                 "\t-n \t\tOptional. Disables GPU reconstruction (not available in ct_amd). Long: --nocuda.\n"
                  "\t-c \t\tOptional. Uses the CPU preprocessing formulas. Long: --cpuPreprocessing.\n"
                  "\t-d 0,1,..,n \tOptional. Sets the GPU devices that shall be used (the first is used).\n"
                  "\t\t\tLong: --cudadevices. Default: 0\n"

@@ -584,7 +584,7 @@ void CtVolume::reconstructVolume() {
         finishReconstruction(CompletionStatus::error(m));
     };
     if (!useCuda_) {
-        fail("Reconstruction runs on the GPU in this build; the CPU path (useCuda = false, -n) is not provided.");
+        fail("Reconstruction runs on the GPU in ct_amd; the CPU path (useCuda = false, -n) is not provided.");
         return;
     }
     if (!cudaAvailable()) {
@@ -592,7 +592,7 @@ void CtVolume::reconstructVolume() {
         return;
     }
     if (activeCudaDevices_.size() > 1)
-        std::cout << "Using device " << activeCudaDevices_[0] << " (one device per reconstruction in this build)."
+        std::cout << "Using device " << activeCudaDevices_[0] << " (one device per reconstruction in ct_amd)."
                   << std::endl;
 
     // projections actually used (projectionStep, Ct/CtVolume.cpp:945)
