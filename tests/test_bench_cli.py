@@ -23,5 +23,17 @@ def test_more_than_one_gpu_is_refused():
 def test_help_lists_workloads():
     r = _run("--help")
     assert r.returncode == 0
-    for name in ("precision2048", "gradient512", "refine768", "--steps", "--warmup"):
+    for name in ("fdk1024", "precision2048", "gradient512", "refine768", "rowfilter1024", "--steps", "--warmup"):
         assert name in r.stdout
+
+
+def test_every_workload_choice_has_a_runner():
+    """The --workload choices are the default FDK line plus exactly the runners of
+    bench_workloads.py (importing it touches no GPU)."""
+    import re
+    sys.path.insert(0, ROOT)
+    import bench_workloads
+    text = open(os.path.join(ROOT, "bench.py")).read()
+    choices = re.search(r'choices=\[("fdk1024"[^\]]*)\]', text).group(1)
+    names = set(re.findall(r'"([a-z0-9]+)"', choices))
+    assert names == {"fdk1024"} | set(bench_workloads.WORKLOADS)
