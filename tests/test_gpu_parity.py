@@ -630,11 +630,12 @@ def test_reconstruct_double_buffered_slabs(gpu):
     assert float(np.abs(zf).max()) > 0.0
 
 
-def test_reconstruct_u8_images_f16_store(gpu):
-    """ctg_reconstruct with 8-bit images (one byte per pixel through the pinned staging)
-    and FP16 entries, against the stream-ordered calls (ctg_preprocess + one
-    ctg_backproject) on the same inputs.  The pipeline backprojects in groups of
-    batches, so the two differ only by FP32 summation order."""
+@pytest.mark.parametrize("pixel,store", [("u8", "f16"), ("f32", "bf16")])
+def test_reconstruct_image_depths_and_stores(gpu, pixel, store):
+    """ctg_reconstruct with 8-bit and 32-bit float images (one and four bytes per pixel
+    through the pinned staging) and FP16 / BF16 entries, against the stream-ordered calls
+    (ctg_preprocess + one ctg_backproject) on the same inputs.  The pipeline may
+    backproject in groups of batches, so the two may differ by FP32 summation order."""
     import torch
     import ct_amd
     W, H, P, n, nz = 128, 96, 24, 64, 48
@@ -644,18 +645,21 @@ def test_reconstruct_u8_images_f16_store(gpu):
     ctx = ct_amd.Context(0)
     ctx.set_geometry(g)
     ctx.set_projections(angles, heights)
-    raw = rng.integers(20, 256, size=(P, H, W)).astype(np.uint8)
+    if pixel == "u8":
+        raw = rng.integers(20, 256, size=(P, H, W)).astype(np.uint8)
+    else:
+        raw = rng.uniform(0.05, 1.0, size=(P, H, W)).astype(np.float32)
     out = np.empty((n, n, nz), np.float32)
-    ctx.reconstruct(raw.ctypes.data, "u8", P, 1.0, "cpu", "hann", "f16", out.ctypes.data, "zfastest")
-    filtered = torch.empty(ctx.filtered_bytes("f16", P), dtype=torch.uint8, device="cuda")
+    ctx.reconstruct(raw.ctypes.data, pixel, P, 1.0, "cpu", "hann", store, out.ctypes.data, "zfastest")
+    filtered = torch.empty(ctx.filtered_bytes(store, P), dtype=torch.uint8, device="cuda")
     scales = torch.empty(P, dtype=torch.float32, device="cuda")
     vol = torch.empty((n, n, nz), dtype=torch.float32, device="cuda")
-    ctx.preprocess(torch.from_numpy(raw).cuda(), "u8", P, 1.0, "cpu", "hann", "f16", filtered, scales)
-    ctx.backproject(filtered, "f16", scales, 0, P, vol, 0, nz)
+    ctx.preprocess(torch.from_numpy(raw).cuda(), pixel, P, 1.0, "cpu", "hann", store, filtered, scales)
+    ctx.backproject(filtered, store, scales, 0, P, vol, 0, nz)
     torch.cuda.synchronize()
     ref = vol.cpu().numpy()
     err = rel_rmse(out, ref)
-    print(f"u8 images, f16 store: pipeline vs stream-ordered calls {err:.2e}")
+    print(f"{pixel} images, {store} store: pipeline vs stream-ordered calls {err:.2e}")
     assert np.isfinite(out).all() and float(np.abs(ref).max()) > 0.0
     assert err < 1e-6
 
