@@ -724,3 +724,22 @@ def test_reconstruct_memory_plans_slabs_and_recomputation(gpu):
     assert e.value.status == _ctg.ERROR_INTERRUPTED
     ctx.reset_cancel()
     assert np.array_equal(recon(), ref)
+    # cancellation during a run, from the progress callback after the first of two
+    # slabs: the run stops with INTERRUPTED at the next slab, and the context then
+    # reconstructs the same volume
+    ctx.set_memory_budget(work + img * 16 * P + slab_col * 32)
+    assert ctx.plan_reconstruct("u16", P, "f32") == (True, 32)
+    calls = []
+
+    def on_progress(frac, user):
+        calls.append(frac)
+        ctx.cancel()
+
+    out = np.empty(n * n * nz, np.float32)
+    with pytest.raises(_ctg.CtgError) as e:
+        ctx.reconstruct(raw.ctypes.data, "u16", P, 1.0, "cpu", "ramlak", "f32", out.ctypes.data, "zfastest",
+                        progress=on_progress)
+    assert e.value.status == _ctg.ERROR_INTERRUPTED
+    assert calls == [0.5]
+    ctx.reset_cancel()
+    assert np.array_equal(recon(), ref)
