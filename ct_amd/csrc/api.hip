@@ -95,6 +95,18 @@ ctg_status fill_u32(void* d, uint32_t value, size_t count, hipStream_t st) {
     return CTG_OK;
 }
 
+ctg_status launch_maxabs(const float* d_images, size_t per_image, int nproj, unsigned* d_maxabs, hipStream_t st) {
+    const unsigned bx = (unsigned)std::min<size_t>((per_image + 255) / 256, 64);
+    // one grid row per image; the grid's y extent is limited to 65535
+    for (int p0 = 0; p0 < nproj; p0 += 65535) {
+        const int np = std::min(65535, nproj - p0);
+        hipLaunchKernelGGL(maxabs_kernel, dim3(bx, (unsigned)np), dim3(256), 0, st, d_images + (size_t)p0 * per_image,
+                           per_image, d_maxabs + p0);
+        CTG_HIP(hipGetLastError());
+    }
+    return CTG_OK;
+}
+
 }  // namespace ctg
 
 using namespace ctg;
@@ -482,12 +494,7 @@ ctg_status ctg_layout_projections(ctg_context* ctx, const float* d_images, int32
     const size_t img = (size_t)ctx->geom.det_width * ctx->geom.det_height;
     CTG_TRY(ctx->maxabs.reserve(sizeof(unsigned) * nproj));
     CTG_TRY(fill_u32(ctx->maxabs.ptr, 0u, (size_t)nproj, st));
-    if (store != CTG_STORE_F32) {
-        const unsigned bx = (unsigned)std::min<size_t>((img + 255) / 256, 64);
-        hipLaunchKernelGGL(maxabs_kernel, dim3(bx, (unsigned)nproj), dim3(256), 0, st, d_images, img,
-                           ctx->maxabs.as<unsigned>());
-        CTG_HIP(hipGetLastError());
-    }
+    if (store != CTG_STORE_F32) CTG_TRY(launch_maxabs(d_images, img, nproj, ctx->maxabs.as<unsigned>(), st));
     return launch_layout(ctx->geom, d_images, nproj, store, ctx->maxabs.as<unsigned>(), d_filtered, d_scales, st);
 }
 
@@ -685,8 +692,9 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
     CTG_TRY(check_range(ctx, 0, nproj));
     if (!h_raw || !h_volume || nproj <= 0) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad buffers");
     if (base_intensity == 0.0f) return fail(CTG_ERROR_INVALID_ARGUMENT, "the base intensity must not be equal to zero");
-    if (bad_filter(filter) || bad_log_form(log_form) || (int)order < 0 || (int)order > 1)
-        return fail(CTG_ERROR_INVALID_ARGUMENT, "bad filter, log form or index order");
+    if (bad_pixel(pixel) || bad_store(store) || bad_filter(filter) || bad_log_form(log_form) || (int)order < 0 ||
+        (int)order > 1)
+        return fail(CTG_ERROR_INVALID_ARGUMENT, "bad pixel type, store, filter, log form or index order");
     const ctg_geometry& g = ctx->geom;
     const size_t img = (size_t)g.det_width * g.det_height;
     const size_t pb = pixel_bytes(pixel);

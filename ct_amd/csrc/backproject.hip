@@ -240,8 +240,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E::kMi
             constexpr bool wide = DMA == 16;
             constexpr int ub = DMA;  // bytes per lane per DMA
             const int units = wide ? (nr * E::kBytes + 15) / 16 : nr * (E::kBytes / 4);
-            // one resource per projection image (its size is below 2^31 bytes, checked on
-            // the host); the column offsets are wave-uniform
+            // one resource per projection image (its size is below 2^31 bytes, checked by
+            // launch_backproject before the launch); the column offsets are wave-uniform
             const int img_bytes = a.W * a.H * E::kBytes;
             const dma_rsrc rs = make_dma_rsrc(reinterpret_cast<const char*>(proj) + (size_t)p * img_bytes, img_bytes);
             const int col0 = (us * a.H + vs) * E::kBytes;
@@ -648,6 +648,13 @@ ctg_status launch_backproject(const BpLaunch& L, hipStream_t st) {
     if (L.nz <= 0 || g.nx <= 0 || g.ny <= 0) return CTG_OK;
     if (((uintptr_t)L.d_filtered % 16) != 0)
         return fail(CTG_ERROR_INVALID_ARGUMENT, "filtered projections must be 16-byte aligned");
+    // The staging DMA addresses one projection through a buffer resource whose size and
+    // column offsets are 32-bit: a projection's entries must stay below 2^31 bytes.
+    const long long img_bytes = (long long)g.det_width * g.det_height * (L.store == CTG_STORE_F32 ? 16 : 8);
+    if (img_bytes > 0x7fffffffll)
+        return fail(CTG_ERROR_UNSUPPORTED,
+                    "detector %dx%d: %lld bytes of backprojection entries per projection exceed the 2^31-byte limit",
+                    g.det_width, g.det_height, img_bytes);
     BpArgs a = make_args(g, L.nproj, L.z0, L.nz, L.accumulate);
     a.ntx = (g.nx + kTX - 1) / kTX;
     a.nty = (g.ny + kTY - 1) / kTY;

@@ -114,6 +114,9 @@ namespace ctg {
 
 // Fill helpers used instead of hipMemsetAsync so that stream capture stays valid.
 ctg_status fill_u32(void* d, uint32_t value, size_t count, hipStream_t st);
+// Per-image max |x| of nproj row-major FP32 images, as float bits, max-combined into
+// d_maxabs (which the caller clears).
+ctg_status launch_maxabs(const float* d_images, size_t per_image, int nproj, unsigned* d_maxabs, hipStream_t st);
 
 ctg_status ensure_filter_tables(ctg_context* ctx, int W, int filter);
 

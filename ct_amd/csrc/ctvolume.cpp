@@ -314,7 +314,8 @@ void CtVolume::setSkipProjections(unsigned value) { projectionStep_ = value + 1;
 
 void CtVolume::stop() {
     stopActiveProcess_ = true;
-    if (ctg_context* c = activeContext_.load()) ctg_cancel(c);
+    std::lock_guard<std::mutex> lock(contextMutex_);
+    if (activeContext_) ctg_cancel(activeContext_);
 }
 
 // ---------------------------------------------------------------- internals
@@ -633,15 +634,24 @@ void CtVolume::reconstructVolume() {
                 if (!mixedDepths_) {
                     memcpy(dst, im.data.data(), npx * bpp);
                 } else {
-                    // mixed depths: intensities as fractions of full scale, 32-bit float
+                    // Mixed depths: intensities as fractions of full scale, 32-bit float.
+                    // For the CPU log form an integer image also takes the 1e-6*B that
+                    // convertTo32bit adds to 8U and 16U images only (Ct/CtVolume.cpp:
+                    // 802-812), evaluated as the GPU does for an integer image,
+                    // fma(I, 1/S, 1e-6*B) in float, so a zero pixel stays finite and the
+                    // values equal those of an unmixed integer stack.
                     float* d = reinterpret_cast<float*>(dst);
                     const double s = pixel_scale(im.depth);
+                    const bool integer = im.depth != PixelDepth::F32;
+                    const float la = (float)(1.0 / s);
+                    const float lb = (float)(0.000001 * (double)baseIntensity_);
                     for (size_t i = 0; i < npx; ++i) {
                         const double v = im.depth == PixelDepth::U8 ? im.data[i]
                                          : im.depth == PixelDepth::U16
                                              ? reinterpret_cast<const uint16_t*>(im.data.data())[i]
                                              : reinterpret_cast<const float*>(im.data.data())[i];
-                        d[i] = (float)(v / s);
+                        if (integer && useCpuPreprocessing_) d[i] = std::fma((float)v, la, lb);
+                        else d[i] = (float)(v / s);
                     }
                 }
             }
@@ -657,8 +667,11 @@ void CtVolume::reconstructVolume() {
         fail(std::string("The GPU could not be initialised: ") + ctg_last_error());
         return;
     }
-    activeContext_ = ctx;
-    if (stopActiveProcess_) ctg_cancel(ctx);
+    {
+        std::lock_guard<std::mutex> lock(contextMutex_);
+        activeContext_ = ctx;
+        if (stopActiveProcess_) ctg_cancel(ctx);
+    }
     const float bounds[6] = {xFrom_float_, xTo_float_, yFrom_float_, yTo_float_, zFrom_float_, zTo_float_};
     ctg_geometry g;
     uint32_t roi[6];
@@ -675,6 +688,11 @@ void CtVolume::reconstructVolume() {
             const double pct = std::round(f * 100.0);
             std::cout << "\r" << "Total completion: " << pct << "%" << std::flush;
             if (self->emitSignals_ && self->reconstructionProgress_) self->reconstructionProgress_(pct);
+            // the slabs finished so far are in the host volume (their downloads have
+            // returned before this call); the rest of it is still zero
+            if (self->emitSignals_ && self->reconstructionCrossSection_)
+                self->reconstructionCrossSection_(pct, self->getVolumeCrossSection(self->crossSectionAxis_,
+                                                                                   self->crossSectionIndex_));
         };
         const ctg_pixel_type pix =
             depth == PixelDepth::U8 ? CTG_PIXEL_U8 : (depth == PixelDepth::U16 ? CTG_PIXEL_U16 : CTG_PIXEL_F32);
@@ -688,7 +706,10 @@ void CtVolume::reconstructVolume() {
                              volume_.data(), CTG_Z_FASTEST, progress, &cb);
     }
     const std::string err = ctg_last_error();
-    activeContext_ = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(contextMutex_);
+        activeContext_ = nullptr;
+    }
     ctg_destroy(ctx);
     if (st == CTG_ERROR_INTERRUPTED || stopActiveProcess_) {
         volume_.clear();
@@ -706,6 +727,89 @@ void CtVolume::reconstructVolume() {
 }
 
 // ------------------------------------------------------------------ writer
+
+// The raw stream of Volume::saveToBinaryFile (Common/Volume.h:474-574): every element
+// as a 32-bit IEEE float in the requested byte order, z fastest (element (x, y, z) at
+// (x*ny + y)*nz + z, the in-memory order) or x fastest ((z*ny + y)*nx + x).  The
+// reference writes element by element through QDataStream; here the stream is cut into
+// chunks of about 32 MiB that worker threads fill (a cache-blocked transpose for X
+// fastest, a byte swap for big endian) while a writer thread writes the previous chunk,
+// so the file write and the reordering overlap.  Returns 0 on success, 1 when the file
+// cannot be opened, 2 on a write error; a set *stop ends the write early (returning 0).
+int writeRawVolume(const std::string& path, const float* zfastest, size_t nx, size_t ny, size_t nz,
+                   IndexOrder order, ByteOrder byteOrder, const std::function<void(double)>& progress,
+                   const std::atomic<bool>* stop) {
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return 1;
+    const bool swap = byteOrder == ByteOrder::BigEndian;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(zfastest);
+    // a chunk is a range of whole x planes (Z fastest) or whole z slices (X fastest)
+    const size_t plane = order == IndexOrder::Z_FASTEST ? ny * nz : nx * ny;
+    const size_t nplanes = order == IndexOrder::Z_FASTEST ? nx : nz;
+    size_t per_chunk = std::max<size_t>(1, ((size_t)32 << 20) / 4 / std::max<size_t>(1, plane));
+    // X fastest reads 16 consecutive z of a column per chunk (one 64-byte line)
+    if (order == IndexOrder::X_FASTEST) per_chunk = std::max<size_t>(16, per_chunk / 16 * 16);
+    const unsigned nt = std::max(1u, std::min<unsigned>(16, std::thread::hardware_concurrency()));
+    std::vector<uint32_t> buf[2];
+    bool ok = true;
+    std::thread writer;
+    auto fill = [&](std::vector<uint32_t>& out, size_t p0, size_t np) {
+        out.resize(np * plane);
+        // work split into units: x planes (Z fastest) or 8-row y bands of the slices (X fastest)
+        const size_t yb = 8;
+        const size_t nby = (ny + yb - 1) / yb;
+        const size_t units = order == IndexOrder::Z_FASTEST ? np : nby;
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t u = next++; u < units; u = next++) {
+                if (order == IndexOrder::Z_FASTEST) {
+                    const uint32_t* s = src + (p0 + u) * plane;
+                    uint32_t* d = out.data() + u * plane;
+                    if (swap)
+                        for (size_t i = 0; i < plane; ++i) d[i] = __builtin_bswap32(s[i]);
+                    else
+                        std::memcpy(d, s, plane * 4);
+                } else {
+                    // slices z in [p0, p0+np), rows y in [y0, y1): tiles of 16 x by the run of z
+                    const size_t y0 = u * yb, y1 = std::min(ny, y0 + yb);
+                    for (size_t x0 = 0; x0 < nx; x0 += 16) {
+                        const size_t x1 = std::min(nx, x0 + 16);
+                        for (size_t y = y0; y < y1; ++y)
+                            for (size_t x = x0; x < x1; ++x) {
+                                const uint32_t* col = src + (x * ny + y) * nz + p0;
+                                uint32_t* d = out.data() + y * nx + x;
+                                for (size_t k = 0; k < np; ++k) {
+                                    const uint32_t w = col[k];
+                                    d[k * plane] = swap ? __builtin_bswap32(w) : w;
+                                }
+                            }
+                    }
+                }
+            }
+        };
+        const unsigned nw = (unsigned)std::min<size_t>(nt, units);
+        std::vector<std::thread> pool;
+        for (unsigned t = 1; t < nw; ++t) pool.emplace_back(work);
+        work();
+        for (auto& t : pool) t.join();
+    };
+    size_t k = 0;
+    for (size_t p0 = 0; p0 < nplanes; p0 += per_chunk, ++k) {
+        if (stop && stop->load()) break;
+        const size_t np = std::min(per_chunk, nplanes - p0);
+        fill(buf[k & 1], p0, np);
+        if (writer.joinable()) writer.join();  // the previous chunk is on disk; its buffer is free
+        if (!ok) break;
+        writer = std::thread([&, k]() {
+            const std::vector<uint32_t>& b = buf[k & 1];
+            if (std::fwrite(b.data(), 4, b.size(), f) != b.size()) ok = false;
+        });
+        if (progress) progress(std::round(double(p0) / double(nplanes) * 100));
+    }
+    if (writer.joinable()) writer.join();
+    const bool closed = std::fclose(f) == 0;
+    return ok && closed ? 0 : 2;
+}
 
 // Ct/CtVolume.cpp:634-721 and Common/Volume.h:474-574
 void CtVolume::saveVolumeToBinaryFile(const std::string& filename, IndexOrder indexOrder, ByteOrder byteOrder) const {
@@ -770,56 +874,24 @@ void CtVolume::saveVolumeToBinaryFile(const std::string& filename, IndexOrder in
         finishSaving(CompletionStatus::error("Did not save the volume, because it appears to be empty."));
         return;
     }
-    FILE* f = std::fopen(filename.c_str(), "wb");
-    if (!f) {
+    auto progress = [this](double pct) {
+        if (emitSignals_ && savingProgress_) savingProgress_(pct);
+    };
+    const int rc = writeRawVolume(filename, volume_.data(), xMax_, yMax_, zMax_, indexOrder, byteOrder, progress,
+                                  &stopActiveProcess_);
+    if (rc == 1) {
         const std::string m = "Could not open the file. Maybe your path does not exist. No files were written.";
         std::cout << m << std::endl;
         finishSaving(CompletionStatus::error(m));
         return;
     }
-    const bool swap = byteOrder == ByteOrder::BigEndian;
-    const size_t nx = xMax_, ny = yMax_, nz = zMax_;
-    auto put = [&](std::vector<uint32_t>& buf) -> bool {
-        if (swap)
-            for (auto& w : buf) w = __builtin_bswap32(w);
-        return std::fwrite(buf.data(), 4, buf.size(), f) == buf.size();
-    };
-    bool ok = true;
-    std::vector<uint32_t> buf;
-    if (indexOrder == IndexOrder::Z_FASTEST) {
-        // in-memory order: x slowest, z fastest
-        const size_t slab = ny * nz;
-        for (size_t x = 0; x < nx && ok; ++x) {
-            if (stopActiveProcess_) break;
-            buf.assign(reinterpret_cast<const uint32_t*>(volume_.data()) + x * slab,
-                       reinterpret_cast<const uint32_t*>(volume_.data()) + (x + 1) * slab);
-            ok = put(buf);
-            if (emitSignals_ && savingProgress_) savingProgress_(std::round(double(x) / double(nx) * 100));
-        }
-    } else {
-        // z slowest, x fastest: gather 16 z-slices per pass
-        const size_t zb = 16;
-        std::vector<uint32_t> slices;
-        for (size_t z0 = 0; z0 < nz && ok; z0 += zb) {
-            if (stopActiveProcess_) break;
-            const size_t nzb = std::min(zb, nz - z0);
-            slices.assign(nzb * nx * ny, 0);
-            for (size_t x = 0; x < nx; ++x)
-                for (size_t y = 0; y < ny; ++y) {
-                    const uint32_t* col = reinterpret_cast<const uint32_t*>(volume_.data()) + (x * ny + y) * nz + z0;
-                    for (size_t k = 0; k < nzb; ++k) slices[(k * ny + y) * nx + x] = col[k];
-                }
-            ok = put(slices);
-            if (emitSignals_ && savingProgress_) savingProgress_(std::round(double(z0) / double(nz) * 100));
-        }
-    }
-    const bool closed = std::fclose(f) == 0;
+    const bool ok = rc == 0;
     if (stopActiveProcess_) {
         std::cout << "User interrupted. Stopping." << std::endl;
         finishSaving(CompletionStatus::interrupted());
         return;
     }
-    if (!ok || !closed) {
+    if (!ok) {
         const std::string m = "An error occured while writing to the disk. Maybe there is not enough free disk space.";
         std::cout << m << std::endl;
         finishSaving(CompletionStatus::error(m));
@@ -886,6 +958,31 @@ int ctv_cross_section(void* v, int axis, unsigned index, float* out, unsigned lo
     if (im.empty()) return 0;
     if (out && cap >= (unsigned long long)im.width * im.height) std::memcpy(out, im.data.data(), im.data.size());
     return 1;
+}
+// Cross-section progress (reconstructionProgress(double, cv::Mat), Ct/CtVolume.h:273):
+// fn(percentage, rows x cols floats of the cross-section at the set axis and index,
+// rows, cols, user) after every finished slab; a null fn removes the callback.
+typedef void (*ctv_cross_section_fn)(double, const float*, int, int, void*);
+void ctv_on_cross_section_progress(void* v, ctv_cross_section_fn fn, void* user) {
+    auto* c = static_cast<ct::CtVolume*>(v);
+    if (!fn) {
+        c->onReconstructionProgress(ct::CtVolume::CrossSectionProgressFn());
+        return;
+    }
+    c->onReconstructionProgress(ct::CtVolume::CrossSectionProgressFn([fn, user](double pct, const ct::Image& im) {
+        fn(pct, im.empty() ? nullptr : reinterpret_cast<const float*>(im.data.data()), im.height, im.width, user);
+    }));
+}
+void ctv_set_cross_section(void* v, int axis, unsigned index) {
+    auto* c = static_cast<ct::CtVolume*>(v);
+    c->setCrossSectionAxis((ct::Axis)axis);
+    c->setCrossSectionIndex(index);
+}
+// writeRawVolume on a caller's Z_FASTEST volume: 0 ok, 1 cannot open, 2 write error.
+int ctv_write_raw(const float* vol, unsigned long long nx, unsigned long long ny, unsigned long long nz,
+                  const char* path, int index_order, int byte_order) {
+    return ct::writeRawVolume(path, vol, nx, ny, nz, (ct::IndexOrder)index_order, (ct::ByteOrder)byte_order, nullptr,
+                              nullptr);
 }
 const float* ctv_volume(void* v, unsigned long long* n) {
     const auto& vol = static_cast<ct::CtVolume*>(v)->volumeData();

@@ -44,9 +44,20 @@ struct Projection {
     double heightOffset = 0.0;
 };
 
+// The raw stream of Volume::saveToBinaryFile (Common/Volume.h:474-574) for a Z_FASTEST
+// volume in memory: 32-bit floats in the given index and byte order.  Returns 0 on
+// success, 1 when the file cannot be opened, 2 on a write error; progress (may be empty)
+// receives percentages, and a set *stop (may be null) ends the write early.
+int writeRawVolume(const std::string& path, const float* zfastest, size_t nx, size_t ny, size_t nz,
+                   IndexOrder order, ByteOrder byteOrder, const std::function<void(double)>& progress,
+                   const std::atomic<bool>* stop);
+
 class CtVolume {
 public:
     using ProgressFn = std::function<void(double percentage)>;
+    // reconstructionProgress(double, cv::Mat crossSection) (Ct/CtVolume.h:273): the
+    // cross-section at the set axis and index of the volume as far as it is finished
+    using CrossSectionProgressFn = std::function<void(double percentage, const Image& crossSection)>;
     using FinishedFn = std::function<void(const CompletionStatus&)>;
 
     CtVolume();
@@ -114,6 +125,10 @@ public:
     void onLoadingProgress(ProgressFn f) { loadingProgress_ = std::move(f); }
     void onLoadingFinished(FinishedFn f) { loadingFinished_ = std::move(f); }
     void onReconstructionProgress(ProgressFn f) { reconstructionProgress_ = std::move(f); }
+    // With a cross-section: called after every finished z-slab (or x range) of the
+    // reconstruction, as the reference's CUDA path emits one after every chunk
+    // (Ct/CtVolume.cpp:1470-1485).  Voxels of slabs not yet finished are 0.
+    void onReconstructionProgress(CrossSectionProgressFn f) { reconstructionCrossSection_ = std::move(f); }
     void onReconstructionFinished(FinishedFn f) { reconstructionFinished_ = std::move(f); }
     void onSavingProgress(ProgressFn f) { savingProgress_ = std::move(f); }
     void onSavingFinished(FinishedFn f) { savingFinished_ = std::move(f); }
@@ -163,9 +178,13 @@ private:
     mutable std::mutex exclusiveFunctionsMutex_;
     std::string lastErrorMessage_;
     mutable CompletionStatus lastStatus_;
-    std::atomic<ctg_context*> activeContext_{nullptr};
+    // the context of a running reconstruction; stop() cancels it under contextMutex_,
+    // and reconstructVolume clears it under the same lock before destroying it
+    std::mutex contextMutex_;
+    ctg_context* activeContext_ = nullptr;
 
     ProgressFn loadingProgress_, reconstructionProgress_;
+    CrossSectionProgressFn reconstructionCrossSection_;
     mutable ProgressFn savingProgress_;
     FinishedFn loadingFinished_, reconstructionFinished_;
     mutable FinishedFn savingFinished_;

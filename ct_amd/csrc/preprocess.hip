@@ -734,18 +734,28 @@ ctg_status launch_preprocess_rows_f64(ctg_context* ctx, const void* d_raw, ctg_p
 
 ctg_status launch_layout(const ctg_geometry& g, const float* d_rows, int nproj, ctg_store store,
                          const unsigned* d_maxabs, void* d_out, float* d_scales, hipStream_t st) {
-    dim3 grid((unsigned)((g.det_width + 31) / 32), (unsigned)((g.det_height + 31) / 32), (unsigned)nproj);
-    if (store == CTG_STORE_F32) {
-        hipLaunchKernelGGL((layout_kernel<CTG_STORE_F32>), grid, dim3(256), 0, st, d_rows, g.det_width,
-                           g.det_height, d_maxabs, d_out, d_scales);
-    } else if (store == CTG_STORE_F16) {
-        hipLaunchKernelGGL((layout_kernel<CTG_STORE_F16>), grid, dim3(256), 0, st, d_rows, g.det_width,
-                           g.det_height, d_maxabs, d_out, d_scales);
-    } else {
-        hipLaunchKernelGGL((layout_kernel<CTG_STORE_BF16>), grid, dim3(256), 0, st, d_rows, g.det_width,
-                           g.det_height, d_maxabs, d_out, d_scales);
+    // one grid z index per projection; the grid's z extent is limited to 65535
+    const size_t img = (size_t)g.det_width * g.det_height;
+    const size_t ebytes = store == CTG_STORE_F32 ? 16 : 8;
+    for (int p0 = 0; p0 < nproj; p0 += 65535) {
+        const int np = std::min(65535, nproj - p0);
+        dim3 grid((unsigned)((g.det_width + 31) / 32), (unsigned)((g.det_height + 31) / 32), (unsigned)np);
+        const float* rows = d_rows + (size_t)p0 * img;
+        const unsigned* mx = d_maxabs + p0;
+        void* out = (char*)d_out + (size_t)p0 * img * ebytes;
+        float* sc = d_scales ? d_scales + p0 : nullptr;
+        if (store == CTG_STORE_F32) {
+            hipLaunchKernelGGL((layout_kernel<CTG_STORE_F32>), grid, dim3(256), 0, st, rows, g.det_width,
+                               g.det_height, mx, out, sc);
+        } else if (store == CTG_STORE_F16) {
+            hipLaunchKernelGGL((layout_kernel<CTG_STORE_F16>), grid, dim3(256), 0, st, rows, g.det_width,
+                               g.det_height, mx, out, sc);
+        } else {
+            hipLaunchKernelGGL((layout_kernel<CTG_STORE_BF16>), grid, dim3(256), 0, st, rows, g.det_width,
+                               g.det_height, mx, out, sc);
+        }
+        CTG_HIP(hipGetLastError());
     }
-    CTG_HIP(hipGetLastError());
     return CTG_OK;
 }
 

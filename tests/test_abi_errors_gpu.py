@@ -87,6 +87,14 @@ def test_enum_values_out_of_range(setup):
     st = lib.ctg_reconstruct(h, host_raw.ctypes.data_as(ctypes.c_void_p), 1, P, 0.9, 0, 5, 0,
                              host_vol.ctypes.data_as(ctypes.c_void_p), 1, _ctg.PROGRESS_FN(), None)
     _rejected(ctx, st, "filter")
+    # an unknown store or pixel type is rejected before the memory plan (a store of 3
+    # would size no filtered buffer; a pixel type of 3 would be read as 4 bytes a pixel)
+    st = lib.ctg_reconstruct(h, host_raw.ctypes.data_as(ctypes.c_void_p), 1, P, 0.9, 0, 0, 3,
+                             host_vol.ctypes.data_as(ctypes.c_void_p), 1, _ctg.PROGRESS_FN(), None)
+    _rejected(ctx, st, "store")
+    st = lib.ctg_reconstruct(h, host_raw.ctypes.data_as(ctypes.c_void_p), 3, P, 0.9, 0, 0, 0,
+                             host_vol.ctypes.data_as(ctypes.c_void_p), 1, _ctg.PROGRESS_FN(), None)
+    _rejected(ctx, st, "pixel")
 
 
 def test_null_buffers_and_ranges(setup):
@@ -152,3 +160,26 @@ def test_row_filter_wider_than_the_lds_buffer_is_unsupported(gpu):
         st = ctx.lib.ctg_preprocess(ctx.handle, _p(raw), 1, 1, 0.9, 0, 0, 0, _p(filtered), None, None)
         assert st == _ctg.ERROR_UNSUPPORTED, (st, _ctg.last_error())
         assert "LDS" in _ctg.last_error()
+
+
+def test_detector_beyond_the_staging_limit_is_unsupported(gpu):
+    """The backprojection stages a projection through a buffer resource with a 32-bit
+    size: 8192 x 16384 FP32 entries (16 B each, 2^31 bytes in one projection) return
+    CTG_ERROR_UNSUPPORTED before any launch and leave the slab untouched."""
+    import torch
+    import ct_amd
+    from ct_amd import _ctg
+    W, H = 8192, 16384
+    g = ct_amd.geometry_grid(W, H, 3.0 * W, 0.0, 16, 16, 8)
+    with ct_amd.Context(0) as ctx:
+        ctx.set_geometry(g)
+        ctx.set_projections([0.0])
+        # the size check comes before any read of the entries: a small buffer stands in
+        filtered = torch.zeros(4096, dtype=torch.uint8, device=gpu)
+        scales = torch.ones(1, dtype=torch.float32, device=gpu)
+        vol = torch.zeros((16, 16, 8), dtype=torch.float32, device=gpu)
+        st = ctx.lib.ctg_backproject(ctx.handle, _p(filtered), 0, _p(scales), 0, 1, _p(vol), 0, 8, 0, None)
+        assert st == _ctg.ERROR_UNSUPPORTED, (st, _ctg.last_error())
+        assert "2^31" in _ctg.last_error()
+        torch.cuda.synchronize()
+        assert bool((vol == 0.0).all())

@@ -258,3 +258,24 @@ def test_cpu_path_is_not_provided(ctv, tmp_path):
     assert f.volume().size == 0
     # no volume: an empty cross section for any axis (Common/Volume.h:578)
     assert f.cross_section(2, 0) is None and f.cross_section(0, 10**6) is None
+
+
+@pytest.mark.parametrize("shape", [(37, 23, 41), (512, 512, 80)])
+def test_raw_writer_orders_and_byte_orders(ctv, tmp_path, shape):
+    """The raw stream of Volume::saveToBinaryFile (Common/Volume.h:474-574): Z fastest
+    keeps the in-memory [x][y][z] order, X fastest is the transpose [z][y][x]; big endian
+    swaps every float.  The 512 x 512 x 80 volume spans several 32 MiB chunks in both
+    orders (the chunks are filled by worker threads and written by another thread)."""
+    ctv.ctv_write_raw.argtypes = [ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_ulonglong, ctypes.c_ulonglong,
+                                  ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+    nx, ny, nz = shape
+    vol = np.random.default_rng(5).standard_normal(shape).astype(np.float32)
+    for order, expect in ((1, vol), (0, np.transpose(vol, (2, 1, 0)))):
+        for bo, dt in ((1, "<f4"), (0, ">f4")):
+            path = str(tmp_path / f"v{order}{bo}.raw")
+            assert ctv.ctv_write_raw(vol.ctypes.data, nx, ny, nz, path.encode(), order, bo) == 0
+            got = np.fromfile(path, dt)
+            assert got.size == vol.size
+            np.testing.assert_array_equal(got, np.ascontiguousarray(expect).ravel())
+            os.remove(path)
+    assert ctv.ctv_write_raw(vol.ctypes.data, nx, ny, nz, str(tmp_path / "no" / "dir.raw").encode(), 1, 1) == 1
