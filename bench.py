@@ -11,8 +11,11 @@ log + Feldkamp weight + ramp-family row filter (ctg_preprocess) and the voxel-dr
 backprojection of all 720 projections into the 1024^3 volume (ctg_backproject).
 value = voxel-updates (nx*ny*nz*P) / step time, in GUPS.  Also reported: the
 backprojection-only rate, the same step with FP16-stored filtered projections, the
-error of both against the FP64 GPU reconstruction, and the CPU oracle's
-backprojection rate on a bounded sub-slab of the same workload (reported only).
+error of both against the FP64 GPU reconstruction, the backprojection's roofline (VALU
+issue, with its measured HBM traffic beside the compulsory bytes), the time to write
+the volume in both index orders, and the CPU restatement of the reference's
+reconstruction timed on the host's cores (a bounded region of the same scan, plus the
+256x192x360 scan in full; reported only).
 """
 from __future__ import annotations
 
@@ -29,6 +32,14 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, 8 TB/s
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "backproject_traffic_1024cubed_720proj.json")
 BYTES_PER_UPDATE = 8.0     # one FP32 voxel load + one store per (voxel, projection)
+# VALU issue peak, non-packed FP32: 256 CUs x 64 lanes x 2.4 GHz, in 1e12 lane-instructions/s
+VALU_PEAK_T = 256 * 64 * 2.4e9 / 1e12
+# lane-instructions per voxel-update of the register-accumulating formulation: one FMA
+# for the detector row, floor and fraction of it, three FMAs for the bilinear form and
+# one FMA into the accumulator (the LDS address is not counted)
+VALU_PER_UPDATE = 7.0
+# LDS bandwidth: 128 bytes per clock per CU, 256 CUs at 2.4 GHz, in TB/s
+LDS_PEAK_TBS = 128 * 256 * 2.4e9 / 1e12
 
 
 def log(msg):
@@ -58,44 +69,137 @@ def build_workload(ctx, torch, args):
     return g, raw, angles, 1.0e4 / 65535.0
 
 
-def cpu_baseline(g, filtered_rows, angles, seconds):
-    """Oracle backprojection (OpenMP, all host threads) on a central z-sub-slab."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
-    import oracle as orc
-
-    og = orc.geometry_from(g)
-    P = filtered_rows.shape[0]
-
-    def run(nz):
-        sub = orc.geometry_from(g)
-        z0 = g.nz // 2 - nz // 2
-        sub.nz = nz
-        sub.z_pre = g.z_pre - z0
-        t0 = time.perf_counter()
-        orc.backproject_f32(sub, filtered_rows, angles)
-        return time.perf_counter() - t0
-
-    t1 = run(1)
-    nz = max(1, min(g.nz, int(seconds / max(t1, 1e-3))))
-    t = run(nz)
-    ups = g.nx * g.ny * nz * P
-    threads = orc.get_threads()
-    # one thread on a single slice (about a tenth of the multi-threaded sample's time)
-    orc.set_threads(1)
-    t_single = run(1)
-    orc.set_threads(0)
-    model = ""
+def _host_cpu():
+    """(logical CPUs usable by this process, model name, SMT state) of the host."""
+    model, smt = "", None
     try:
         with open("/proc/cpuinfo") as f:
             model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
     except OSError:
         pass
-    return {"value": ups / t / 1e9, "unit": "GUPS", "cores": threads, "kind": "port",
-            "sample": f"CPU oracle backprojection (reconstructionCore restatement, OpenMP) of {P} projections "
-                      f"into a {g.nx}x{g.ny}x{nz} central z-slab of the same grid, {t:.1f} s wall",
-            "single_thread_gups": g.nx * g.ny * P / t_single / 1e9,
-            "cpu_model": model, "host_logical_cpus": os.cpu_count()}
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            smt = f.read().strip() == "1"
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count()
+    return n, model, smt
+
+
+def _timing_build():
+    """The oracle's source built at -O3 -march=native on this host (oracle/Makefile's
+    `timing` target) into a temporary directory; None if the build fails."""
+    import subprocess
+    import tempfile
+    out = tempfile.mkdtemp(prefix="oracle_timing_")
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "timing", f"OUT={out}"],
+                       capture_output=True, text=True)
+    path = os.path.join(out, "liboracle_timing.so")
+    return path if r.returncode == 0 and os.path.exists(path) else None
+
+
+def cpu_baseline(torch, raw2, base2, angles2, g2):
+    """The CPU restatement of the reference's reconstruction (CPU preprocessing:
+    Ct/CtVolume.cpp:787-885; reconstructionCore: :923-1054, OpenMP dynamic over x)
+    timed on all host threads with the -O3 -march=native build: the 256x192 scan with
+    360 projections on CtVolume's 254x254x192 grid in full, and the 1024^2 x 720 scan
+    on CtVolume's grid bounded to x, y in [0.25, 0.75], z in [0.1, 0.9] (501x501x820),
+    both from the raw uint16 images.  `value` is the second: the bounded sample of the
+    GPU workload's scan.  Also: the rate on one thread (8 slices of the first)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as orc
+    import ct_amd
+    from ct_amd.phantom import circular_angles, phantom_unit_radius, random_ellipsoids
+
+    orc.load()
+    fast = _timing_build()
+    build = "g++ -O3 -march=native -fopenmp (oracle.c)" if fast else "pinned oracle build (-O2)"
+    if fast:
+        orc.use(fast)
+    try:
+        # 256x192 detector, FCD 1250 px, 360 projections, 10 ellipsoids, I = round(65535 e^-p)
+        W1, H1, P1, fcd1 = 256, 192, 360, 1250.0
+        g1, _ = ct_amd.geometry_ctvolume(W1, H1, fcd1, 0.0, None)
+        ang1 = circular_angles(P1)
+        unit = phantom_unit_radius(g1.radius, H1, fcd1)
+        ell = [e[:9] + [e[9] / unit] for e in random_ellipsoids(10, 1, unit)]
+        with ct_amd.Context(0) as c1:
+            c1.set_geometry(g1)
+            c1.set_projections(ang1)
+            r1 = torch.empty((P1, H1, W1), dtype=torch.int16, device="cuda")
+            c1.phantom_images(ell, 0, P1, 0.0, 65535.0, 1, r1, None)
+            raw1 = r1.cpu().numpy().view(np.uint16)
+        og1 = orc.geometry_from(g1)
+        t0 = time.perf_counter()
+        orc.fdk_f32(og1, raw1, "u16", 1.0, "ramlak", ang1)
+        t1 = time.perf_counter() - t0
+        ups1 = float(g1.nx) * g1.ny * g1.nz * P1
+        # one thread: the same scan's backprojection into 8 central slices
+        filt1 = orc.preprocess_f32(og1, raw1, "u16", 1.0, "ramlak", "cpu")
+        sub = orc.geometry_from(g1)
+        sub.nz = 8
+        sub.z_pre = g1.z_pre - (g1.nz // 2 - 4)
+        threads = orc.get_threads()
+        orc.set_threads(1)
+        t0 = time.perf_counter()
+        orc.backproject_f32(sub, filt1, ang1)
+        ts = time.perf_counter() - t0
+        orc.set_threads(0)
+        # the GPU workload's scan on the bounded CtVolume grid
+        g2r, roi = ct_amd.geometry_ctvolume(g2.det_width, g2.det_height, g2.fcd, 0.0, [0.25, 0.75, 0.25, 0.75, 0.1, 0.9])
+        host2 = raw2.view(torch.int16).cpu().numpy().view(np.uint16)
+        t0 = time.perf_counter()
+        orc.fdk_f32(orc.geometry_from(g2r), host2, "u16", base2, "shepplogan", angles2)
+        t2 = time.perf_counter() - t0
+        ups2 = float(g2r.nx) * g2r.ny * g2r.nz * len(angles2)
+    finally:
+        orc.use(None)
+    n, model, smt = _host_cpu()
+    return {"value": ups2 / t2 / 1e9, "unit": "GUPS", "cores": threads, "kind": "port",
+            "sample": f"CPU restatement of the reference's FDK (CPU preprocessing + reconstructionCore, OpenMP) of "
+                      f"the same 720x1024x1024 scan on CtVolume's grid bounded to x,y in [0.25,0.75], z in "
+                      f"[0.1,0.9] ({g2r.nx}x{g2r.ny}x{g2r.nz}), from the raw uint16 images: {t2:.1f} s wall",
+            "build": build, "cpu_model": model, "host_logical_cpus": n, "smt_active": smt,
+            "roi_seconds": t2,
+            "cfg256_grid": [g1.nx, g1.ny, g1.nz], "cfg256_seconds": t1, "cfg256_gups": ups1 / t1 / 1e9,
+            "cfg256_workload": f"{P1}x{H1}x{W1} uint16, ramlak, CtVolume grid {g1.nx}x{g1.ny}x{g1.nz}, "
+                               f"preprocessing + backprojection",
+            "single_thread_gups": float(sub.nx) * sub.ny * sub.nz * P1 / ts / 1e9}
+
+
+def write_timing(vol_host, nx, ny, nz):
+    """ms to write the volume's raw stream (libctvolume's writer: Z fastest, X fastest,
+    little endian) next to one plain fwrite of the same bytes (the host's write rate,
+    the bound for the writer), into a temporary file that is removed afterwards."""
+    import ctypes
+    import tempfile
+    lib = ctypes.CDLL(os.path.join(ROOT, "ct_amd", "libctvolume.so"))
+    lib.ctv_write_raw.argtypes = [ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_ulonglong, ctypes.c_ulonglong,
+                                  ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+    d = tempfile.mkdtemp(prefix="ct_write_")
+    path = os.path.join(d, "vol.raw")
+    out = {}
+    try:
+        for name, order in (("zfastest", 1), ("xfastest", 0)):
+            t0 = time.perf_counter()
+            rc = lib.ctv_write_raw(vol_host.ctypes.data, nx, ny, nz, path.encode(), order, 1)
+            out[f"ms_write_{name}"] = (time.perf_counter() - t0) * 1e3
+            os.remove(path)
+            if rc != 0:
+                raise RuntimeError(f"volume write failed ({rc})")
+        t0 = time.perf_counter()
+        with open(path, "wb") as f:
+            f.write(memoryview(vol_host).cast("B"))
+        out["ms_write_plain_fwrite"] = (time.perf_counter() - t0) * 1e3
+        os.remove(path)
+    finally:
+        os.rmdir(d)
+    out["write_bytes"] = int(vol_host.nbytes)
+    return out
 
 
 def main():
@@ -108,7 +212,7 @@ def main():
     ap.add_argument("--grid", type=int, default=None, help="cubic grid size (default: the workload's)")
     ap.add_argument("--no-precision", action="store_true", help="skip the FP64 reference and FP16 error")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-write", action="store_true", help="skip the volume write timing")
     ap.add_argument("--only", choices=["f32", "f16", "both"], default="both")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end run from pinned host memory")
     ap.add_argument("--no-roi", action="store_true", help="skip the region-of-interest run")
@@ -118,6 +222,8 @@ def main():
     ap.add_argument("--det-w", type=int, default=0, help="detector width (other workloads)")
     ap.add_argument("--det-h", type=int, default=0, help="detector height (other workloads)")
     ap.add_argument("--slab", type=int, default=0, help="z-slab height (other workloads)")
+    ap.add_argument("--checkpoint-dir", default="",
+                    help="refine768: write per-slab checkpoints to this (empty) directory")
     ap.add_argument("--calib-copy-gib", type=float, default=0.0,
                     help="run a device-to-device copy of this size first (HBM counter calibration)")
     args = ap.parse_args()
@@ -287,11 +393,11 @@ def main():
     e2e = None if args.no_e2e else e2e_from_pinned_host("f32")
     h2h = None if args.no_e2e else reconstruct_host_to_host()
     vol32 = vol.clone() if not args.no_precision else None
-    rows_for_cpu = None
-    if not args.no_cpu_baseline:
-        # filtered rows (first tap of each pair) for the CPU oracle
-        rows_for_cpu = filt32.view(torch.float32).view(P, g.det_width, g.det_height, 4)[..., 0] \
-            .transpose(1, 2).contiguous().cpu().numpy()
+    writes = None
+    if not args.no_write:
+        writes = write_timing(vol.cpu().numpy(), g.nx, g.ny, g.nz)
+        log(f"volume write ({writes['write_bytes'] / 2**30:.1f} GiB): Z fastest {writes['ms_write_zfastest']:.0f} ms, "
+            f"X fastest {writes['ms_write_xfastest']:.0f} ms, one plain fwrite {writes['ms_write_plain_fwrite']:.0f} ms")
     del filt32, sc32
     if args.only in ("f16", "both"):
         t16, bp16, f16buf, sc16 = run_store("f16")
@@ -319,18 +425,26 @@ def main():
         del vol64
 
     cpu = None
-    if rows_for_cpu is not None:
+    if not args.no_cpu_baseline:
         log("CPU baseline ...")
-        cpu = cpu_baseline(g, rows_for_cpu, angles, args.cpu_seconds)
-        log(f"CPU: {cpu['value']:.3f} GUPS on {cpu['cores']} threads")
+        cpu = cpu_baseline(torch, raw, base, angles, g)
+        log(f"CPU: {cpu['value']:.3f} GUPS on {cpu['cores']} threads ({cpu['build']}); 254x254x192 from "
+            f"360x192x256 in {cpu['cfg256_seconds']:.2f} s")
 
     t32, bp32 = res["f32"]
-    achieved = BYTES_PER_UPDATE * updates / bp32 / 1e9
+    # The FP32 backprojection keeps its voxels in registers, so its binder is the VALU
+    # issue rate, not HBM: achieved = VALU_PER_UPDATE lane-instructions per voxel-update
+    # (the formulation's count, see the constant) / kernel time, against the non-packed
+    # FP32 issue peak of 256 CUs x 64 lanes x 2.4 GHz.
+    achieved = VALU_PER_UPDATE * updates / bp32 / 1e12
     traffic = None
     if os.path.exists(TRAFFIC_FILE) and (args.det, args.nproj, args.grid) == (1024, 720, 1024):
         # HBM bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated
         # against a copy kernel of known size (see the profile summary beside the file)
         traffic = json.load(open(TRAFFIC_FILE))["traffic_bytes_per_launch"]
+    # compulsory HBM bytes of one launch: every FP32 entry (16 B a texel) read once and
+    # the volume written once
+    compulsory = 16.0 * g.det_width * g.det_height * P + 4.0 * g.nx * g.ny * g.nz
     out = {
         "metric": "FDK voxel-update rate (backprojection GUPS), 1024^3 from 720x1024^2; FP32 vs FP16-stored projections",
         "value": updates / t32 / 1e9,
@@ -350,8 +464,18 @@ def main():
         "ms_per_recon_f32": t32 * 1e3,
         "bp_ms_f32": bp32 * 1e3,
         "bp_gups_f32": updates / bp32 / 1e9,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_T, "unit": "T lane-instr/s",
+                     "frac": achieved / VALU_PEAK_T, "traffic": traffic, "compulsory_bytes": compulsory,
+                     "traffic_over_compulsory": traffic / compulsory if traffic else None,
+                     "valu_per_update": VALU_PER_UPDATE},
+        # the other on-chip ceiling: one 16-byte FP32 entry read from LDS per update,
+        # against 128 B per clock per CU (256 CUs at 2.4 GHz)
+        "roofline_lds": {"bound": "lds", "achieved": 16.0 * updates / bp32 / 1e12, "peak": LDS_PEAK_TBS,
+                         "unit": "TB/s", "frac": 16.0 * updates / bp32 / 1e12 / LDS_PEAK_TBS, "bytes_per_update": 16},
+        # the 8-byte read-modify-write model of the metric's definition (the reference
+        # kernel's per-update voxel load and store, Ct/cuda_ct.cu:193-199): 1000 GUPS
+        # at 8 TB/s, against which a 60 % HBM-roofline target reads 600 GUPS
+        "hbm_8b_per_update_frac": BYTES_PER_UPDATE * updates / bp32 / 1e9 / HBM_PEAK_GBS,
         "hbm_copy_measured_gbs": copy_gbs,
         "cpu_baseline": cpu,
     }
@@ -366,6 +490,8 @@ def main():
     if not args.no_roi:
         out.update(roi_run(ctx, torch, raw, P, base, filt, stream, args.steps))
     out.update(prec)
+    if writes:
+        out.update(writes)
     print(json.dumps(out), flush=True)
 
 

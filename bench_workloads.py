@@ -13,8 +13,10 @@ gradient512    512^3 volume, 360 projections of 512x512 (FCD 2500 px): forward p
 refine768      sparse view: 120 projections of 1024x768 (FCD 5000 px), 768^3 volume,
                128 seeded ellipsoids, Poisson counts at I0 = 5e3.  FDK seed from FP16
                filtered projections, then 10 steepest-descent steps (exact line search) on
-               0.5 ||A x - p||^2 with the adjoint pair, in z-slabs; RMSE against the
-               voxelised analytic phantom and against the plain FP32 FDK, ms per step.
+               0.5 ||A x - p||^2 with the adjoint pair, in z-slabs (ct_amd.refine; with
+               --checkpoint-dir every finished slab of each step is checkpointed there);
+               RMSE against the voxelised analytic phantom and against the plain FP32
+               FDK, ms per step.
 rowfilter1024  the ramp-family row filter over 720 x 1024 rows of 1024 (Shepp-Logan):
                ctg_filter_rows (the register FFT) against the same circular filter as a
                dense circulant GEMM on the matrix cores (torch.matmul with FP32 inputs,
@@ -26,6 +28,7 @@ from __future__ import annotations
 
 import json
 import math
+import os
 import sys
 import time
 
@@ -168,6 +171,7 @@ def refine768(args):
     import ct_amd
     from ct_amd.autograd import ConeBeamOperator
     from ct_amd.phantom import circular_angles, phantom_unit_radius, random_ellipsoids, voxelize
+    from ct_amd.refine import refine
 
     dev = torch.device("cuda:0")
     W, H, P, n = args.det_w or 1024, args.det_h or 768, args.nproj or 120, args.grid or 768
@@ -219,28 +223,29 @@ def refine768(args):
     ls_scale = float(torch.sum(ax.double() * lineint.double()) / torch.sum(ax.double() * ax.double()))
     del ax
     hist = [{"step": 0, "rmse_truth": rmse(x, truth), "rmse_vs_fp32_fdk": rmse(x, fdk["f32"])}]
-    step_ms = []
-    for it in range(1, 11):
-        _sync(torch)
-        t0 = time.perf_counter()
-        r = op.project(x) - lineint
-        grad = op.backproject(r)
-        ag = op.project(grad)
-        alpha = float(torch.sum(grad.double() * grad.double()) / torch.sum(ag.double() * ag.double()))
-        x -= alpha * grad
-        _sync(torch)
-        step_ms.append((time.perf_counter() - t0) * 1e3)
-        loss = 0.5 * float(torch.sum(r.double() * r.double()))
-        hist.append({"step": it, "loss_before": loss, "rmse_truth": rmse(x, truth),
-                     "rmse_vs_fp32_fdk": rmse(x, fdk["f32"])})
-        log(f"step {it}: loss {loss:.4e}, RMSE vs phantom {hist[-1]['rmse_truth']:.4e}, {step_ms[-1]:.0f} ms")
+    step_ms, ck_ms = [], []
+
+    def on_step(rec, xk):
+        step_ms.append(rec["ms"])
+        ck_ms.append(rec["ms_checkpoint"])
+        hist.append({"step": rec["step"], "loss_before": rec["loss_before"], "rmse_truth": rmse(xk, truth),
+                     "rmse_vs_fp32_fdk": rmse(xk, fdk["f32"])})
+        log(f"step {rec['step']}: loss {rec['loss_before']:.4e}, RMSE vs phantom {hist[-1]['rmse_truth']:.4e}, "
+            f"{rec['ms']:.0f} ms (+{rec['ms_checkpoint']:.0f} ms checkpoint writes)")
+
+    ckdir = getattr(args, "checkpoint_dir", "") or None
+    if ckdir and os.path.exists(os.path.join(ckdir, "state.json")):
+        raise SystemExit(f"{ckdir} holds a checkpoint of an earlier run; give an empty directory")
+    x, _ = refine(op, x, lineint, 10, checkpoint_dir=ckdir, on_step=on_step)
     out = {"workload": f"sparse-view refinement: {n}^3 from {P}x{H}x{W}, Poisson I0=5e3, FP16 FDK seed, "
-                       f"10 steepest-descent steps in z-slabs of {op.slab_nz}",
+                       f"10 steepest-descent steps in z-slabs of {op.slab_nz}"
+                       + (", per-slab checkpoints" if ckdir else ""),
            "fdk_scale_to_mu_per_voxel": scale, "fdk_seed_least_squares_scale": ls_scale,
            "rmse_ls_scaled_fdk_vs_truth": rmse(fdk["f16"] * ls_scale, truth),
            "rmse_fp32_fdk_vs_truth": rmse(fdk["f32"], truth), "rmse_fp16_fdk_vs_truth": rmse(fdk["f16"], truth),
            "truth_rms": float(torch.sqrt(torch.mean(truth[mask].double() ** 2))),
-           "ms_per_step": sum(step_ms) / len(step_ms), "steps": hist}
+           "ms_per_step": sum(step_ms) / len(step_ms),
+           "ms_checkpoint_per_step": sum(ck_ms) / len(ck_ms) if ckdir else None, "steps": hist}
     print(json.dumps(out), flush=True)
 
 

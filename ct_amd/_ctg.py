@@ -91,6 +91,8 @@ SIGNATURES = {
     "ctg_feldkamp_weight": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p]),
     "ctg_filter_rows": (c_int32, [c_void_p, c_void_p, c_void_p, c_int64, c_int32, c_void_p]),
     "ctg_layout_projections": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_void_p]),
+    "ctg_layout_projections_noalloc": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_void_p, c_void_p,
+                                                 c_void_p]),
     "ctg_backproject": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_int32, c_void_p, c_int32,
                                   c_int32, c_int32, c_void_p]),
     "ctg_forward_project": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int32, c_void_p, c_int32,

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <thread>
 #include <vector>
@@ -110,6 +111,13 @@ ctg_status launch_maxabs(const float* d_images, size_t per_image, int nproj, uns
 }  // namespace ctg
 
 using namespace ctg;
+
+// Max-abs counters of ctg_layout_projections_noalloc: a static device array, one copy
+// per GPU.  In host code `scratch` names a host-side stand-in, not the device array, so
+// kernels and fills get the per-device address that hipGetSymbolAddress returns,
+// resolved once per device with that device current (noalloc_scratch_for).
+constexpr int kNoAllocMaxProj = CTG_NOALLOC_MAX_PROJ;
+static __device__ __align__(256) unsigned scratch[kNoAllocMaxProj];
 
 namespace {
 
@@ -226,6 +234,22 @@ void parallel_ranges(size_t n, size_t min_per_thread, F fn) {
     }
     fn(0, std::min(n, per));
     for (auto& t : pool) t.join();
+}
+
+std::mutex g_scratch_mu;
+std::vector<unsigned*> g_scratch_ptr;  // indexed by device
+
+// Address of `scratch` on `device`, which must be the current device.
+ctg_status noalloc_scratch_for(int device, unsigned** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        CTG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[device] = static_cast<unsigned*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return CTG_OK;
 }
 
 }  // namespace
@@ -353,9 +377,12 @@ ctg_status ctg_create(int32_t device, ctg_context** out) {
     }
     if (device < 0 || device >= n) return fail(CTG_ERROR_NO_DEVICE, "device %d not available (%d devices)", device, n);
     CTG_HIP(hipSetDevice(device));
+    unsigned* sp = nullptr;
+    CTG_TRY(noalloc_scratch_for(device, &sp));
     ctg_context* c = new (std::nothrow) ctg_context();
     if (!c) return fail(CTG_ERROR_OUT_OF_MEMORY, "host allocation failed");
     c->device = device;
+    c->noalloc_scratch = sp;
     *out = c;
     return CTG_OK;
 }
@@ -381,6 +408,7 @@ ctg_status ctg_destroy(ctg_context* ctx) {
     }
     ctx->slab_conv.release();
     ctx->ell.release();
+    ctx->fp_work.release();
     delete ctx;
     return CTG_OK;
 }
@@ -498,6 +526,26 @@ ctg_status ctg_layout_projections(ctg_context* ctx, const float* d_images, int32
     return launch_layout(ctx->geom, d_images, nproj, store, ctx->maxabs.as<unsigned>(), d_filtered, d_scales, st);
 }
 
+ctg_status ctg_layout_projections_noalloc(ctg_context* ctx, const float* d_images, int32_t nproj, ctg_store store,
+                                          void* d_filtered, float* d_scales, void* stream) {
+    clear_error();
+    CTG_TRY(check_ctx(ctx));
+    if (bad_store(store)) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad store");
+    if (nproj < 0 || (nproj > 0 && (!d_images || !d_filtered))) return fail(CTG_ERROR_INVALID_ARGUMENT, "bad buffers");
+    if (nproj > kNoAllocMaxProj)
+        return fail(CTG_ERROR_INVALID_ARGUMENT, "%d projections exceed the %d of the allocation-free layout", nproj,
+                    kNoAllocMaxProj);
+    CTG_TRY(check_scales(store, d_scales, nproj));
+    if (nproj == 0) return CTG_OK;
+    if (!ctx->noalloc_scratch) return fail(CTG_ERROR_DEVICE, "the static max-abs buffer was not resolved");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t img = (size_t)ctx->geom.det_width * ctx->geom.det_height;
+    unsigned* mx = ctx->noalloc_scratch;  // the device address of `scratch`, never the host stand-in
+    CTG_TRY(fill_u32(mx, 0u, (size_t)nproj, st));
+    if (store != CTG_STORE_F32) CTG_TRY(launch_maxabs(d_images, img, nproj, mx, st));
+    return launch_layout(ctx->geom, d_images, nproj, store, mx, d_filtered, d_scales, st);
+}
+
 ctg_status ctg_backproject(ctg_context* ctx, const void* d_filtered, ctg_store store, const float* d_scales,
                            int32_t p0, int32_t nproj, float* d_volume, int32_t z0, int32_t nz, int32_t accumulate,
                            void* stream) {
@@ -539,7 +587,8 @@ ctg_status ctg_forward_project(ctg_context* ctx, const float* d_volume, int32_t 
     hipStream_t st = (hipStream_t)stream;
     const size_t n = (size_t)nproj * ctx->geom.det_width * ctx->geom.det_height;
     if (!accumulate) CTG_TRY(fill_u32(d_proj, 0u, n, st));
-    return launch_forward_project(ctx->geom, ctx->proj_dev.as<float4>() + p0, d_volume, z0, nz, nproj, d_proj, st);
+    return launch_forward_project(ctx->geom, ctx->proj_dev.as<float4>() + p0, ctx->proj_host.data() + p0, d_volume, z0,
+                                  nz, nproj, d_proj, ctx->fp_work, st);
 }
 
 ctg_status ctg_fdk_backward(ctg_context* ctx, const float* d_grad_volume, int32_t z0, int32_t nz, int32_t p0,
@@ -555,8 +604,8 @@ ctg_status ctg_fdk_backward(ctg_context* ctx, const float* d_grad_volume, int32_
     const ctg_geometry& g = ctx->geom;
     const size_t n = (size_t)nproj * g.det_width * g.det_height;
     CTG_TRY(fill_u32(d_grad_lineint, 0u, n, st));
-    CTG_TRY(launch_forward_project(g, ctx->proj_dev.as<float4>() + p0, d_grad_volume, z0, nz, nproj, d_grad_lineint,
-                                   st));
+    CTG_TRY(launch_forward_project(g, ctx->proj_dev.as<float4>() + p0, ctx->proj_host.data() + p0, d_grad_volume, z0,
+                                   nz, nproj, d_grad_lineint, ctx->fp_work, st));
     CTG_TRY(launch_filter_rows_only(ctx, d_grad_lineint, d_grad_lineint, (int64_t)nproj * g.det_height, filter, st));
     return launch_preprocess_rows(ctx, d_grad_lineint, CTG_PIXEL_F32, nproj, 1.0f, CTG_LOG_CPU_FORM, filter,
                                   d_grad_lineint, nullptr, false, true, false, st);

@@ -108,6 +108,8 @@ struct ctg_context {
     ctg::HostBuf pin_up[2];          // pinned staging of raw images (pipeline)
     ctg::HostBuf pin_dl[2];          // pinned download chunks (pipeline)
     ctg::DevBuf ell;                 // phantom ellipsoids
+    ctg::DevBuf fp_work;             // column partials of the forward projector
+    unsigned* noalloc_scratch = nullptr;  // the device's static max-abs counters (ctg_layout_projections_noalloc)
 };
 
 namespace ctg {
@@ -151,8 +153,12 @@ ctg_status launch_backproject_f64(const ctg_geometry& g, const double4* d_params
                                   int nproj, double* d_volume, int z0, int nz, hipStream_t st);
 
 // ---- project.hip
-ctg_status launch_forward_project(const ctg_geometry& g, const float4* d_params, const float* d_volume,
-                                  int z0, int nz, int nproj, float* d_proj, hipStream_t st);
+// Adds the forward projection of the slab into d_proj (row-major nproj x H x W images).
+// h_params: host copy of the projections' parameters (row bounds of the entry-space
+// form; null selects the gather form); work: the context buffer for its column partials.
+ctg_status launch_forward_project(const ctg_geometry& g, const float4* d_params, const float4* h_params,
+                                  const float* d_volume, int z0, int nz, int nproj, float* d_proj, DevBuf& work,
+                                  hipStream_t st);
 ctg_status launch_phantom(const ctg_geometry& g, const double4* d_params, const double* h_ell,
                           int n_ell, int nproj, int p0, double photons, double full_scale,
                           uint64_t seed, uint16_t* d_img, float* d_lineint, DevBuf& ell_buf,

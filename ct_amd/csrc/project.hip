@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 
 namespace ctg {
@@ -374,6 +375,319 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
     }
 }
 
+// -------------------------------------------------------- entry-space projector
+//
+// The backprojection evaluates a voxel's value from ONE coefficient entry (A, dU, dV, dUV)
+// at (u0, v0): val = A + fu dU + fv (dV + fu dUV), weighted by w = U^2.  Its transpose
+// therefore adds (w x, w x fu, w x fv, w x fu fv) into one entry gradient per (voxel,
+// projection), and the transpose of the layout (A = q00, dU = q10 - q00, dV = q01 - q00,
+// dUV = q11 - q10 - q01 + q00) turns entry gradients into pixels:
+//     q00 += gA - gdU - gdV + gdUV,  q01 += gdV - gdUV,  q10 += gdU - gdUV,  q11 += gdUV.
+// Every voxel is visited once per projection (by the workgroup of its own u0), with one
+// LDS read and one write of 16 bytes, instead of twice with two row taps each.
+//
+// fp_entry_kernel<G>: a workgroup owns entry column c of one projection within one
+// window of 64 G slices (grid z [wz0, wz0 + 64 G), wz0 - zrb a multiple of 64).
+//  - Candidates are the voxel columns with u0 == c that pass the cylinder and u window
+//    tests, between the two fan rays through u = c - 0.05 and u = c + 1.05 (plus a
+//    0.01-voxel margin).  The lanes test the wedge's (depth, lateral) slots and compact
+//    the survivors (column, U, fu) into LDS; then one lane per survivor settles its z
+//    interval of the v-window test (exactly, with the backprojection's expression),
+//    intersected with the slab and the window, and writes the final record.
+//  - A candidate is one item of G chunks of 64 slices: lane l takes slices
+//    wz0 + 64 k + l, whose row anchor is (l & 32) slices past the chunk start and whose
+//    offset from it is l & 31, as in the backprojection.  The voxels come through a
+//    buffer resource covering the candidate's interval (zero outside it, which is the
+//    window, slab and v-window mask).
+//  - Each wave accumulates into two private copies of the window's entry rows (even and
+//    odd lanes).  Lanes two slices apart have rows at least 2U - 1e-3 > 1 apart (U > 2/3
+//    whenever the cylinder radius is below half the FCD, checked on the host), and chunks
+//    are 64 slices apart, so no two lanes of one copy touch one entry in an instruction
+//    and the read-modify-writes need no atomics.  The host bounds the rows that any slice
+//    of the window can reach for every candidate U and height, [rbase, rbase + span), so
+//    lanes outside their interval (adding zero) stay inside their own copy.
+//  - At the end the eight copies are summed in a fixed order and turned into pixel
+//    partials: own column c (q00, q01) and the next column (q10, q11), added into the
+//    workgroup's own float2 column ps[p][c][*] of a column-major buffer.  Windows run one
+//    after another on one stream; fp_finish_kernel then adds ps[p][c][r].x +
+//    ps[p][c-1][r].y into the row-major image.
+// The order of every addition is fixed: the result does not depend on scheduling.
+constexpr int kESlots = 512;
+constexpr int kEMaxSpan = 256;  // entry rows per copy
+
+struct FpWin {
+    int wz0;     // first grid slice of the window (a multiple of 64 past the run base)
+    int rbase;   // first entry row any voxel of the window can reach
+    int span;    // rows per copy
+};
+
+__device__ inline int flr_i32(float x) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+template <int G>
+__global__ __launch_bounds__(kGThreads) void fp_entry_kernel(FpArgs a, FpWin win, const float4* __restrict__ pp,
+                                                             const float* __restrict__ vol,
+                                                             float2* __restrict__ ps) {
+    __shared__ int4 slots[kESlots];      // column, U, w, w fu
+    __shared__ int slot_z[kESlots];      // z interval relative to the run base
+    __shared__ int wave_cnt[kGWaves];
+    extern __shared__ float4 acc_lds[];  // [wave][row][parity]
+    const int c = blockIdx.x;
+    const int p = blockIdx.y;
+    const int tid = (int)threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int span = win.span;
+    for (int i = tid; i < 2 * kGWaves * span; i += kGThreads) acc_lds[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+
+    const float4 g = pp[p];
+    const float sn = g.x, cs = g.y, h = g.z;
+    const float Sx = a.fcd * cs + a.uoff * sn, Sy = a.fcd * sn - a.uoff * cs;
+    const float ulo = (float)c - 0.05f - a.upre, uhi = (float)c + 1.05f - a.upre;
+    const float q0x = -(ulo - a.uoff) * sn, q0y = (ulo - a.uoff) * cs;
+    const float q1x = -(uhi - a.uoff) * sn, q1y = (uhi - a.uoff) * cs;
+    const float umid = (float)c + 0.5f - a.upre - a.uoff;
+    const bool xmajor = fabsf(-umid * sn - Sx) >= fabsf(umid * cs - Sy);
+    const int n_out = xmajor ? a.nx : a.ny, n_in = xmajor ? a.ny : a.nx;
+    const float pre_out = xmajor ? a.xpre : a.ypre, pre_in = xmajor ? a.ypre : a.xpre;
+    const float So = xmajor ? Sx : Sy, Si = xmajor ? Sy : Sx;
+    const float k0 = ((xmajor ? q0y : q0x) - Si) / ((xmajor ? q0x : q0y) - So);
+    const float k1 = ((xmajor ? q1y : q1x) - Si) / ((xmajor ? q1x : q1y) - So);
+    const float R = sqrtf(a.r2);
+    const int o_lo = max(0, (int)floorf(pre_out - R) - 1), o_hi = min(n_out - 1, (int)ceilf(pre_out + R) + 1);
+    const int zend = a.z0 + a.nzs - 1;
+    const int zrb = a.z0 & ~(kRowAnchor - 1);
+    // the slices of this window inside the slab
+    const int wlo = max(a.z0, win.wz0), whi = min(zend, win.wz0 + 64 * G - 1);
+
+    auto jrange = [&](int o, int& jl, int& jh) {
+        const float ow = (float)o - pre_out;
+        const float i0 = Si + (ow - So) * k0, i1 = Si + (ow - So) * k1;
+        jl = max(0, (int)ceilf(fminf(i0, i1) + pre_in - 0.01f));
+        jh = min(n_in - 1, (int)floorf(fmaxf(i0, i1) + pre_in + 0.01f));
+    };
+    // lateral slots per depth step: the wedge width |i1 - i0| is linear in the depth and
+    // the source lies outside the volume, so it peaks at an end of the depth range; a
+    // window of width w + 0.02 holds at most floor(w) + 2 integers
+    auto width = [&](int o) {
+        const float ow = (float)o - pre_out;
+        return fabsf((ow - So) * (k1 - k0));
+    };
+    const int jm = (int)fmaxf(width(o_lo), width(o_hi)) + 2;
+    const int nslot = (o_hi - o_lo + 1) * jm;
+
+    // per lane and chunk: world z of the slice's row anchor plus the height (the
+    // backprojection's zwa + h), and the slice's offset from the anchor
+    float zh[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) zh[k] = ((float)(win.wz0 + 64 * k + (lane & 32)) - a.zpre) + h;
+    const float lane_k = (float)(lane & 31);
+    // LDS byte address of entry row v0 in this lane's copy: v0 * 32 + lane_base
+    const int lane_base = ((wv * span) * 2 + (lane & 1)) * 16 - win.rbase * 32;
+    char* acc_bytes = reinterpret_cast<char*>(acc_lds);
+    const float* colbase = vol + (zrb - a.z0);
+    const int zc = win.wz0 - zrb;  // the window's first slice relative to the run base
+    const unsigned lane4 = (unsigned)lane * (unsigned)sizeof(float);
+    __syncthreads();
+
+    int ncand = 0;
+    for (int sbase = 0; sbase < nslot && wlo <= whi; sbase += kGThreads) {
+        // fill, step 1: the cheap tests for 256 slots, survivors compacted (ballot prefix
+        // within each wave, wave totals through LDS)
+        const int sidx = sbase + tid;
+        int4 rec = make_int4(-1, 0, 0, 0);
+        if (sidx < nslot) {
+            const int oi = sidx / jm, jj = sidx - oi * jm;
+            const int o = o_lo + oi;
+            int jl, jh;
+            jrange(o, jl, jh);
+            const int j = jl + jj;
+            if (j <= jh) {
+                const int xi = xmajor ? o : j, yi = xmajor ? j : o;
+                // per-voxel arithmetic identical to the backprojection's
+                const float x = (float)xi - a.xpre, y = (float)yi - a.ypre;
+                if (x * x + y * y < a.r2) {
+                    float t = (-1.0f) * x * g.x + y * g.y;
+                    t += a.uoff;
+                    const float s = x * g.y + y * g.x;
+                    const float U = cone_magnification(a.fcd, s);
+                    const float u = t * U;
+                    if (u >= a.ulo && u <= a.uhi) {
+                        const float uIdx = u + a.upre;
+                        const int u0 = (int)uIdx;
+                        if (u0 == c) rec = make_int4(xi * a.ny + yi, __float_as_int(U), __float_as_int(uIdx - (float)u0), 0);
+                    }
+                }
+            }
+        }
+        {
+            const unsigned long long m = __ballot(rec.x >= 0);
+            const int below = __popcll(m & ((1ull << lane) - 1ull));
+            if (lane == 0) wave_cnt[wv] = __popcll(m);
+            __syncthreads();
+            int base = ncand;
+            for (int w = 0; w < wv; ++w) base += wave_cnt[w];
+            int total = 0;
+            for (int w = 0; w < kGWaves; ++w) total += wave_cnt[w];
+            if (rec.x >= 0) slots[base + below] = rec;
+            __syncthreads();  // the slots are complete; wave_cnt is rewritten below
+            // fill, step 2: one lane per new survivor settles its z interval and writes its
+            // final record (candidates with an empty interval keep an empty one)
+            const int ci = ncand + tid;
+            if (tid < total) {
+                const int4 r = slots[ci];
+                const float U = __int_as_float(r.y), fu = __int_as_float(r.z);
+                auto v_of = [&](int zi) { const float zk = (float)zi - a.zpre; return (zk + h) * U; };
+                int lo = wlo, hi = whi;
+                if (!(v_of(wlo) >= a.vlo && v_of(whi) <= a.vhi)) {
+                    // the passing z of v = (zk + h) U in [vLo, vHi] form an interval (v is
+                    // non-decreasing in z): estimate its ends, then settle them with the
+                    // backprojection's own expression
+                    const float iU = __builtin_amdgcn_rcpf(U);
+                    const float zlo_f = fminf(fmaxf(a.vlo * iU - h + a.zpre, (float)(wlo - 4)), (float)(whi + 4));
+                    const float zhi_f = fminf(fmaxf(a.vhi * iU - h + a.zpre, (float)(wlo - 4)), (float)(whi + 4));
+                    lo = (int)ceilf(zlo_f);
+                    hi = (int)floorf(zhi_f);
+#pragma unroll
+                    for (int it = 0; it < 2; ++it) {
+                        if (lo > wlo && v_of(lo - 1) >= a.vlo) --lo;
+                        if (hi < whi && v_of(hi + 1) <= a.vhi) ++hi;
+                    }
+#pragma unroll
+                    for (int it = 0; it < 2; ++it) {
+                        if (lo <= whi && !(v_of(lo) >= a.vlo)) ++lo;
+                        if (hi >= wlo && !(v_of(hi) <= a.vhi)) --hi;
+                    }
+                    lo = max(lo, wlo);
+                    hi = min(hi, whi);
+                }
+                const float w = U * U;
+                slots[ci] = make_int4(r.x, r.y, __float_as_int(w), __float_as_int(w * fu));
+                // an empty interval [lo, hi] with hi < lo becomes a zero-length resource
+                slot_z[ci] = (lo - zrb) | (max(hi, lo - 1) - zrb + 1) << kFpZBits;
+            }
+            ncand += total;
+            __syncthreads();  // records complete; wave_cnt free again
+        }
+        // walk when another fill round might not fit, and after the last round
+        if (ncand > kESlots - kGThreads || sbase + kGThreads >= nslot) {
+            struct Item {
+                vbuf rs;
+                float U, w, wfu;
+                int lo;
+            };
+            // past the last candidate an item still issues its loads, over an empty
+            // resource (they return zero without touching memory)
+            auto item_of = [&](int ci, bool valid) {
+                const int4 r = slots[valid ? ci : 0];
+                const int zb = __builtin_amdgcn_readfirstlane(slot_z[valid ? ci : 0]);
+                Item it;
+                it.lo = zb & ((1 << kFpZBits) - 1);
+                const int hi1 = (zb >> kFpZBits) & ((1 << kFpZBits) - 1);  // one past the last slice
+                it.rs = make_vbuf(colbase + (size_t)__builtin_amdgcn_readfirstlane(r.x) * a.zstride + it.lo,
+                                  valid ? (hi1 - it.lo) * (int)sizeof(float) : 0);
+                it.U = __int_as_float(__builtin_amdgcn_readfirstlane(r.y));
+                it.w = __int_as_float(__builtin_amdgcn_readfirstlane(r.z));
+                it.wfu = __int_as_float(__builtin_amdgcn_readfirstlane(r.w));
+                return it;
+            };
+            // Candidate ci goes to wave ci mod 4.  Three named slots keep the next two
+            // candidates' voxels in flight while one is accumulated.
+            struct Slot { Item it; float x[G]; };
+            int ci_f = wv;
+            auto next = [&](Slot& sl) -> bool {
+                const bool valid = ci_f < ncand;
+                sl.it = item_of(ci_f, valid);
+#pragma unroll
+                for (int k = 0; k < G; ++k)
+                    sl.x[k] = vbuf_load(sl.it.rs, vadd_sv((zc + 64 * k - sl.it.lo) * (int)sizeof(float), lane4));
+                ci_f += kGWaves;
+                __builtin_amdgcn_sched_barrier(0);  // keep the slots' loads in walk order
+                return valid;
+            };
+            auto process = [&](const Slot& sl) {
+#pragma unroll
+                for (int k = 0; k < G; ++k) {
+                    const float vA = fmaf(-zh[k], sl.it.U, a.vpre);  // det_row_anchor(zwa, h, U, vpre)
+                    const float vI = det_row_index(lane_k, sl.it.U, vA);
+                    const int v0 = flr_i32(vI);
+                    const float fv = __builtin_amdgcn_fractf(vI);
+                    const float t = sl.it.w * sl.x[k], tu = sl.it.wfu * sl.x[k];
+                    float4* e = reinterpret_cast<float4*>(acc_bytes + (v0 * 32 + lane_base));
+                    float4 v = *e;
+                    v.x += t;
+                    v.y += tu;
+                    v.z = fmaf(t, fv, v.z);
+                    v.w = fmaf(tu, fv, v.w);
+                    *e = v;
+                }
+            };
+            Slot sa, sb, sc;
+            bool ha = next(sa), hb = next(sb), hc = next(sc);
+            while (ha) {
+                process(sa);
+                ha = next(sa);
+                if (!hb) break;
+                process(sb);
+                hb = next(sb);
+                if (!hc) break;
+                process(sc);
+                hc = next(sc);
+            }
+            ncand = 0;
+            __syncthreads();  // every wave is done with the slots
+        }
+    }
+    // sum the eight copies per entry row (waves in order, even before odd) and turn the
+    // entry gradients into pixel partials: own column (q00 at row v, q01 at v + 1) and
+    // the next column (q10 at v, q11 at v + 1); pixel row rbase + r takes entry rows r
+    // and r - 1
+    float2* col = ps + ((size_t)p * a.W + c) * a.H;
+    auto entry = [&](int r) {
+        float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (r >= 0 && r < span) {
+            for (int w = 0; w < kGWaves; ++w)
+                for (int par = 0; par < 2; ++par) {
+                    const float4 e = acc_lds[(w * span + r) * 2 + par];
+                    sum.x += e.x; sum.y += e.y; sum.z += e.z; sum.w += e.w;
+                }
+        }
+        return sum;
+    };
+    for (int r = tid; r <= span; r += kGThreads) {
+        const int row = win.rbase + r;
+        if (row < 0 || row >= a.H) continue;
+        const float4 e0 = entry(r), e1 = entry(r - 1);  // entries at rows v = row and v = row - 1
+        const float own = ((e0.x - e0.y) - e0.z + e0.w) + (e1.z - e1.w);
+        const float side = (e0.y - e0.w) + e1.w;
+        float2 o = col[row];
+        o.x += own;
+        o.y += side;
+        col[row] = o;
+    }
+}
+
+// img[p][r][c] += ps[p][c][r].x + ps[p][c - 1][r].y: column-major partials into the
+// row-major images, through a 32 x 33 tile per (row, column) block.
+__global__ __launch_bounds__(256) void fp_finish_kernel(const float2* __restrict__ ps, int W, int H,
+                                                        float* __restrict__ img) {
+    __shared__ float2 tile[33][33];  // [column - c0 + 1][row - r0]
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const size_t p = blockIdx.z;
+    const int tl = threadIdx.x & 31;
+    for (int i = threadIdx.x >> 5; i < 33; i += 8) {
+        const int cc = c0 - 1 + i, r = r0 + tl;
+        tile[i][tl] = (cc >= 0 && cc < W && r < H) ? ps[(p * W + cc) * H + r] : make_float2(0.0f, 0.0f);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x >> 5; i < 32; i += 8) {
+        const int r = r0 + i, cc = c0 + tl;
+        if (cc < W && r < H) img[(p * H + r) * W + cc] += tile[tl + 1][i].x + tile[tl][i].y;
+    }
+}
+
 // ------------------------------------------------------------------ phantom
 
 struct EllDev {
@@ -501,8 +815,28 @@ __global__ __launch_bounds__(256) void copy_swap_kernel(const float* __restrict_
 
 }  // namespace
 
-ctg_status launch_forward_project(const ctg_geometry& g, const float4* d_params, const float* d_volume, int z0,
-                                  int nz, int nproj, float* d_proj, hipStream_t st) {
+namespace {
+
+// Entry rows that the slices [wz0, wz0 + nsl) of any voxel column can reach: vIndex =
+// vPre - (z + h) U over every world z of the window, every height of the launch's
+// projections and every magnification of the cylinder, with two rows of margin for the
+// rounding of the kernel's anchored expression.
+void window_rows(const ctg_geometry& g, double hmin, double hmax, double Umin, double Umax, int wz0, int nsl,
+                 int& rbase, int& span) {
+    const double ylo = (double)wz0 - g.z_pre + hmin, yhi = (double)(wz0 + nsl - 1) - g.z_pre + hmax;
+    const double c[4] = {ylo * Umin, ylo * Umax, yhi * Umin, yhi * Umax};
+    const double mn = std::min(std::min(c[0], c[1]), std::min(c[2], c[3]));
+    const double mx = std::max(std::max(c[0], c[1]), std::max(c[2], c[3]));
+    rbase = (int)std::floor(g.v_pre - mx) - 2;
+    const int rtop = (int)std::floor(g.v_pre - mn) + 2;
+    span = rtop - rbase + 1;
+}
+
+}  // namespace
+
+ctg_status launch_forward_project(const ctg_geometry& g, const float4* d_params, const float4* h_params,
+                                  const float* d_volume, int z0, int nz, int nproj, float* d_proj, DevBuf& work,
+                                  hipStream_t st) {
     if (nz <= 0 || nproj <= 0) return CTG_OK;
     FpArgs a{};
     a.W = g.det_width; a.H = g.det_height; a.nx = g.nx; a.ny = g.ny; a.nzs = nz; a.z0 = z0; a.nproj = nproj;
@@ -525,6 +859,57 @@ ctg_status launch_forward_project(const ctg_geometry& g, const float4* d_params,
         return fail(CTG_ERROR_UNSUPPORTED, "forward projection needs the v window inside the detector rows");
     dim3 grid((unsigned)a.W, (unsigned)nproj);
     if (grid.y > 65535u) return fail(CTG_ERROR_UNSUPPORTED, "forward projection grid too large");
+
+    // Entry-space form (fp_entry_kernel) when the windows' rows fit the LDS copies: it
+    // needs U > 2/3 (radius below half the FCD) and host parameters for the row bounds.
+    int G = 0;
+    double hmin = 0.0, hmax = 0.0;
+    const double R = g.radius, F = g.fcd;
+    const double Umin = F / (F + R) * (1.0 - 1e-5), Umax = F / (F - R) * (1.0 + 1e-5);
+    const bool entry_ok = h_params != nullptr && R < 0.5 * F && !std::getenv("CTG_FP_GATHER") &&
+                          std::getenv("CTG_FP_ENTRY");
+    if (entry_ok) {
+        hmin = hmax = h_params[0].z;
+        for (int i = 1; i < nproj; ++i) {
+            hmin = std::min(hmin, (double)h_params[i].z);
+            hmax = std::max(hmax, (double)h_params[i].z);
+        }
+        const int zrb = z0 & ~(kRowAnchor - 1);
+        for (int cand : {4, 2, 1}) {
+            bool fits = true;
+            for (int w0 = zrb; w0 < z0 + nz && fits; w0 += 64 * cand) {
+                int rb, sp;
+                window_rows(g, hmin, hmax, Umin, Umax, w0, 64 * cand, rb, sp);
+                fits = sp <= kEMaxSpan;
+            }
+            if (fits) {
+                G = cand;
+                break;
+            }
+        }
+    }
+    if (G > 0 && nz <= kFpMaxSlab) {
+        const size_t cells = (size_t)nproj * a.W * a.H;
+        CTG_TRY(work.reserve(cells * sizeof(float2)));
+        float2* ps = work.as<float2>();
+        CTG_TRY(fill_u32(ps, 0u, cells * 2, st));
+        const int zrb = z0 & ~(kRowAnchor - 1);
+        for (int w0 = zrb; w0 < z0 + nz; w0 += 64 * G) {
+            FpWin win{};
+            win.wz0 = w0;
+            window_rows(g, hmin, hmax, Umin, Umax, w0, 64 * G, win.rbase, win.span);
+            const size_t lds = sizeof(float4) * 2 * kGWaves * (size_t)win.span;
+            if (G == 4) hipLaunchKernelGGL(fp_entry_kernel<4>, grid, dim3(kGThreads), lds, st, a, win, d_params, d_volume, ps);
+            else if (G == 2) hipLaunchKernelGGL(fp_entry_kernel<2>, grid, dim3(kGThreads), lds, st, a, win, d_params, d_volume, ps);
+            else hipLaunchKernelGGL(fp_entry_kernel<1>, grid, dim3(kGThreads), lds, st, a, win, d_params, d_volume, ps);
+            CTG_HIP(hipGetLastError());
+        }
+        dim3 fg((unsigned)((a.W + 31) / 32), (unsigned)((a.H + 31) / 32), (unsigned)nproj);
+        hipLaunchKernelGGL(fp_finish_kernel, fg, dim3(256), 0, st, ps, a.W, a.H, d_proj);
+        CTG_HIP(hipGetLastError());
+        return CTG_OK;
+    }
+
     const size_t lds = sizeof(float) * (size_t)(2 * kGWaves) * (a.H + 4);
     // the candidate records hold z relative to the launch's run base in kFpZBits bits
     for (int s = 0; s < nz; s += kFpMaxSlab) {

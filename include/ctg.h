@@ -179,6 +179,25 @@ ctg_status ctg_layout_projections(ctg_context* ctx, const float* d_images, int32
                                   ctg_store store, void* d_filtered, float* d_scales,
                                   void* stream);
 
+/* Maximum projection count of one ctg_layout_projections_noalloc call. */
+#define CTG_NOALLOC_MAX_PROJ 65536
+
+/* ctg_layout_projections without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc), e.g. to lay out the
+ * images of an A^T application that is captured whole (ctg_backproject allocates
+ * nothing).  The per-projection max-abs counters that ctg_layout_projections keeps in
+ * context memory live here in a library-owned static device buffer of
+ * CTG_NOALLOC_MAX_PROJ entries (256 KiB, held on every GPU the library runs on until
+ * the process ends); its address on the context's device is resolved by ctg_create.
+ * nproj above CTG_NOALLOC_MAX_PROJ returns CTG_ERROR_INVALID_ARGUMENT.  The counters
+ * are cleared by a fill kernel in every call; calls that overlap on one GPU (other
+ * streams, other contexts, other host threads) share the buffer and race.  Results
+ * are identical to ctg_layout_projections'. */
+ctg_status ctg_layout_projections_noalloc(ctg_context* ctx, const float* d_images, int32_t nproj,
+                                          ctg_store store, void* d_filtered, float* d_scales,
+                                          void* stream);
+
 /* Voxel-driven backprojection of projections [p0, p0+nproj) (geometry indices
  * from ctg_set_projections; d_filtered holds them in order starting with p0)
  * into the slab d_volume = z-range [z0, z0+nz).  accumulate = 0 overwrites the

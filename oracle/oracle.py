@@ -34,47 +34,63 @@ class Geometry(Structure):
 
 
 _lib = None
+_pinned = None
 _D = POINTER(c_double)
 _F = POINTER(c_float)
 
 
-def load():
+def use(path=None):
+    """Routes the functions below to another build of oracle.c at `path` (bench.py's
+    timing build, -O3 -march=native); None returns to the pinned build."""
     global _lib
+    if path is None:
+        _lib = _pinned
+        load()
+    else:
+        _lib = _configure(ctypes.CDLL(path))
+    return _lib
+
+
+def load():
+    global _lib, _pinned
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise OSError(f"{LIB_PATH} missing: run `make -C oracle`")
-        lib = ctypes.CDLL(LIB_PATH)
-        sig = {
-            "orc_cylinder_radius": (c_uint32, [c_uint32, c_float]),
-            "orc_volume_geometry": (c_int, [c_uint32, c_uint32, c_float, c_float, _F, POINTER(Geometry),
-                                            POINTER(c_uint32), _F]),
-            "orc_grid_geometry": (None, [c_uint32, c_uint32, c_float, c_float, c_int32, c_int32, c_int32,
-                                         POINTER(Geometry)]),
-            "orc_log_cpu": (None, [c_void_p, c_int, c_size_t, c_double, _F]),
-            "orc_log_gpu": (None, [c_void_p, c_int, c_size_t, c_double, _F]),
-            "orc_log_cpu_f64": (None, [c_void_p, c_int, c_size_t, c_double, _D]),
-            "orc_feldkamp": (None, [_F, c_int32, c_int32, c_int32, c_float, c_float, c_float]),
-            "orc_feldkamp_f64": (None, [_D, c_int32, c_int32, c_int32, c_double, c_double, c_double]),
-            "orc_filter_window": (None, [c_int, c_int32, _F]),
-            "orc_filter_rows": (None, [_F, c_int32, c_int64, c_int]),
-            "orc_filter_rows_f64": (None, [_D, c_int32, c_int64, c_int]),
-            "orc_filter_rows_direct_f64": (None, [_D, c_int32, c_int64, c_int]),
-            "orc_backproject_f32": (None, [POINTER(Geometry), _F, _D, _D, c_int32, c_int32, _F]),
-            "orc_backproject_f64": (None, [POINTER(Geometry), _D, _D, _D, c_int32, _D]),
-            "orc_forward_project_f64": (None, [POINTER(Geometry), _D, _D, _D, c_int32, _D]),
-            "orc_fdk_f32": (None, [POINTER(Geometry), c_void_p, c_int, c_double, c_int, c_int, _D, _D, c_int32, _F]),
-            "orc_fdk_f64": (None, [POINTER(Geometry), c_void_p, c_int, c_double, c_int, _D, _D, c_int32, _D]),
-            "orc_preprocess_f32": (None, [POINTER(Geometry), c_void_p, c_int, c_double, c_int, c_int, c_int32, _F]),
-            "orc_phantom_project": (None, [POINTER(Geometry), _D, c_int32, _D, _D, c_int32, _D]),
-            "orc_set_threads": (None, [c_int]),
-            "orc_get_threads": (c_int, []),
-        }
-        for name, (res, args) in sig.items():
-            fn = getattr(lib, name)
-            fn.restype = res
-            fn.argtypes = args
-        _lib = lib
+        _lib = _pinned = _configure(ctypes.CDLL(LIB_PATH))
     return _lib
+
+
+def _configure(lib):
+    sig = {
+        "orc_cylinder_radius": (c_uint32, [c_uint32, c_float]),
+        "orc_volume_geometry": (c_int, [c_uint32, c_uint32, c_float, c_float, _F, POINTER(Geometry),
+                                        POINTER(c_uint32), _F]),
+        "orc_grid_geometry": (None, [c_uint32, c_uint32, c_float, c_float, c_int32, c_int32, c_int32,
+                                     POINTER(Geometry)]),
+        "orc_log_cpu": (None, [c_void_p, c_int, c_size_t, c_double, _F]),
+        "orc_log_gpu": (None, [c_void_p, c_int, c_size_t, c_double, _F]),
+        "orc_log_cpu_f64": (None, [c_void_p, c_int, c_size_t, c_double, _D]),
+        "orc_feldkamp": (None, [_F, c_int32, c_int32, c_int32, c_float, c_float, c_float]),
+        "orc_feldkamp_f64": (None, [_D, c_int32, c_int32, c_int32, c_double, c_double, c_double]),
+        "orc_filter_window": (None, [c_int, c_int32, _F]),
+        "orc_filter_rows": (None, [_F, c_int32, c_int64, c_int]),
+        "orc_filter_rows_f64": (None, [_D, c_int32, c_int64, c_int]),
+        "orc_filter_rows_direct_f64": (None, [_D, c_int32, c_int64, c_int]),
+        "orc_backproject_f32": (None, [POINTER(Geometry), _F, _D, _D, c_int32, c_int32, _F]),
+        "orc_backproject_f64": (None, [POINTER(Geometry), _D, _D, _D, c_int32, _D]),
+        "orc_forward_project_f64": (None, [POINTER(Geometry), _D, _D, _D, c_int32, _D]),
+        "orc_fdk_f32": (None, [POINTER(Geometry), c_void_p, c_int, c_double, c_int, c_int, _D, _D, c_int32, _F]),
+        "orc_fdk_f64": (None, [POINTER(Geometry), c_void_p, c_int, c_double, c_int, _D, _D, c_int32, _D]),
+        "orc_preprocess_f32": (None, [POINTER(Geometry), c_void_p, c_int, c_double, c_int, c_int, c_int32, _F]),
+        "orc_phantom_project": (None, [POINTER(Geometry), _D, c_int32, _D, _D, c_int32, _D]),
+        "orc_set_threads": (None, [c_int]),
+        "orc_get_threads": (c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
 
 
 def _p(a, ctype):

@@ -37,3 +37,27 @@ def test_every_workload_choice_has_a_runner():
     choices = re.search(r'choices=\[("fdk1024"[^\]]*)\]', text).group(1)
     names = set(re.findall(r'"([a-z0-9]+)"', choices))
     assert names == {"fdk1024"} | set(bench_workloads.WORKLOADS)
+
+
+def test_cpu_baseline_timing_build_matches_the_pinned_oracle():
+    """The CPU baseline times oracle.c built at -O3 -march=native on the measuring host;
+    that build reconstructs a small scan as the pinned (-O2, no contraction) build does,
+    to float rounding."""
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import bench
+    import oracle as orc
+    from helpers import rel_rmse, small_scan
+    scan = small_scan(W=48, H=32, P=20, seed=3)
+    _, g, _, _ = orc.volume_geometry(48, 32, scan["fcd"], 0.0, None)
+    ref = orc.fdk_f32(g, scan["raw"], "u16", 1.0, "hann", scan["angles"])
+    path = bench._timing_build()
+    assert path is not None
+    try:
+        orc.use(path)
+        got = orc.fdk_f32(g, scan["raw"], "u16", 1.0, "hann", scan["angles"])
+    finally:
+        orc.use(None)
+    assert np.isfinite(got).all()
+    assert rel_rmse(got, ref) < 1e-5

@@ -540,6 +540,57 @@ def test_forward_projector_wide_cone_and_slab_widths(gpu, oracle, nz):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("case", ["ctvolume", "wide_cone_heights", "slab_offset_uoffset", "tall"])
+def test_forward_projector_entry_form_matches_gather_form(gpu, case):
+    """The forward projector runs in entry space (one visit per voxel, windows of 64 or
+    128 slices) wherever the windows' rows fit its LDS copies, and falls back to the
+    pixel-space gather otherwise (CTG_FP_GATHER=1 forces the gather).  Both transpose
+    the same backprojection, so they agree to FP32 rounding; accumulation into a filled
+    image adds exactly the projection."""
+    import os
+    import torch
+    import ct_amd
+    rng = np.random.default_rng(21)
+    if case == "ctvolume":
+        g, _ = ct_amd.geometry_ctvolume(96, 80, 480.0, 0.0, None)
+        P, z0, nz, heights = 16, 0, 80, None
+    elif case == "wide_cone_heights":
+        g = ct_amd.geometry_grid(64, 48, 65.0, 0.0, 40, 40, 200)
+        P, z0, nz, heights = 10, 0, 200, np.linspace(-2.0, 2.0, 10)
+    elif case == "slab_offset_uoffset":
+        g = ct_amd.geometry_grid(80, 72, 400.0, 2.5, 60, 60, 300)
+        P, z0, nz, heights = 12, 37, 150, np.linspace(-3.0, 1.0, 12)
+    else:
+        g = ct_amd.geometry_grid(24, 1024, 300.0, 0.0, 12, 12, 1000)
+        P, z0, nz, heights = 5, 0, 1000, np.array([0.0, 3.5, -7.25, 1.0, 2.0])
+    angles = np.sort(rng.uniform(0.0, 360.0, P))
+    ctx = ct_amd.Context(0)
+    ctx.set_geometry(g)
+    ctx.set_projections(angles, heights)
+    x = torch.from_numpy(rng.standard_normal((g.nx, g.ny, nz)).astype(np.float32)).cuda()
+    H, W = g.det_height, g.det_width
+    out = {}
+    for form in ("entry", "gather"):
+        if form == "gather":
+            os.environ["CTG_FP_GATHER"] = "1"
+        try:
+            Ax = torch.empty((P, H, W), dtype=torch.float32, device="cuda")
+            ctx.forward_project(x, z0, nz, 0, P, Ax)
+            torch.cuda.synchronize()
+        finally:
+            os.environ.pop("CTG_FP_GATHER", None)
+        out[form] = Ax.cpu().numpy()
+    err = rel_rmse(out["entry"], out["gather"])
+    print(f"{case}: entry vs gather form rel RMSE {err:.2e}")
+    assert float(np.abs(out["gather"]).max()) > 0.0
+    assert err < 2e-6
+    base = torch.from_numpy(rng.standard_normal((P, H, W)).astype(np.float32)).cuda()
+    acc = base.clone()
+    ctx.forward_project(x, z0, nz, 0, P, acc, accumulate=True)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(acc.cpu().numpy(), base.cpu().numpy() + out["entry"], rtol=1e-6, atol=1e-4)
+
+
 def test_phantom_generator_matches_oracle(gpu, oracle):
     import torch
     scan = small_scan(W=64, H=48, P=6, seed=12)
