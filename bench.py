@@ -233,6 +233,17 @@ def main():
               "(FDK would shard over z-slabs with no exchange step). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if args.calib_copy_gib > 0:
+        # a device copy of known size first, for calibrating the HBM byte counters
+        import torch
+        n = int(args.calib_copy_gib * 2**30) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {n * 4} bytes read and written, twice")
+        del src, dst
     if args.workload != "fdk1024":
         import bench_workloads
         bench_workloads.WORKLOADS[args.workload](args)
@@ -248,15 +259,6 @@ def main():
     torch.cuda.set_device(dev)
     info = ct_amd._ctg.device_info(0)
     log(f"device: {info.name.decode()} CUs={info.compute_units} HBM={info.total_bytes / 2**30:.0f} GiB")
-    if args.calib_copy_gib > 0:
-        n = int(args.calib_copy_gib * 2**30) // 4
-        src = torch.ones(n, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(2):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"calibration copy: {n * 4} bytes read and written, twice")
-        del src, dst
     # measured HBM copy rate beside the published 8 TB/s: 1 GiB read and 1 GiB written
     # per copy, timed with events over 5 copies after one warm-up
     n = (1 << 30) // 4

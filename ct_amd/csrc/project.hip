@@ -866,8 +866,7 @@ ctg_status launch_forward_project(const ctg_geometry& g, const float4* d_params,
     double hmin = 0.0, hmax = 0.0;
     const double R = g.radius, F = g.fcd;
     const double Umin = F / (F + R) * (1.0 - 1e-5), Umax = F / (F - R) * (1.0 + 1e-5);
-    const bool entry_ok = h_params != nullptr && R < 0.5 * F && !std::getenv("CTG_FP_GATHER") &&
-                          std::getenv("CTG_FP_ENTRY");
+    const bool entry_ok = h_params != nullptr && R < 0.5 * F && !std::getenv("CTG_FP_GATHER");
     if (entry_ok) {
         hmin = hmax = h_params[0].z;
         for (int i = 1; i < nproj; ++i) {
