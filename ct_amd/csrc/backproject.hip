@@ -55,7 +55,6 @@ struct BpArgs {
 // Entry formats: FP32 (16 B), FP16 and BF16 (8 B: {A,dU} {dV,dUV} as 16-bit pairs).
 struct EF32 {
     using T = float4;
-    static constexpr bool kFix = false;  // float row index (see EF16X)
     static constexpr int kBytes = 16;
     static constexpr int kMinWaves = 1;  // LDS limits FP32 tiles to 3 workgroups per CU anyway
     static constexpr int kGrp = 4;       // LDS reads per software-pipeline group
@@ -65,7 +64,6 @@ struct EF32 {
 };
 struct EF16 {
     using T = uint2;
-    static constexpr bool kFix = false;
     static constexpr int kBytes = 8;
     static constexpr int kMinWaves = 4;  // VALU-bound: keep 4 waves per SIMD (at most 128 VGPRs)
     static constexpr int kGrp = 2;       // (groups of 4 need spills at 128 VGPRs: measured slower)
@@ -77,7 +75,6 @@ struct EF16 {
 };
 struct EBF16 {
     using T = uint2;
-    static constexpr bool kFix = false;
     static constexpr int kBytes = 8;
     static constexpr int kMinWaves = 4;
     static constexpr int kGrp = 2;
@@ -87,16 +84,6 @@ struct EBF16 {
         return fmaf(fv, fmaf(fu, hi(e.y), lo(e.y)), fmaf(fu, hi(e.x), lo(e.x)));
     }
 };
-
-// FP16 / BF16 entries read with a 16.16 fixed-point row index (fast path): the integer
-// part carries the LDS entry index, so the address is one shift of its upper half and the
-// fraction fv * 2^16 is one conversion of its lower half (both single SDWA instructions),
-// instead of floor, fraction and an address add.  The entries then hold (A, dU) times 2^14
-// and (dV, dUV) times 2^-2 (relative to the per-projection scale), so that
-// A' + fu dU' + fvraw (dV' + fu dUV') = 2^14 (A + fu dU + fv (dV + fu dUV)), and the weight
-// carries 2^-14.  Interpolation and accumulation stay FP32.
-struct EF16X : EF16 { static constexpr bool kFix = true; };
-struct EBF16X : EBF16 { static constexpr bool kFix = true; };
 
 // Global->LDS DMA through a buffer resource: B bytes (4 or 16) per lane from byte
 // offset soff + voff of the resource into lds + lane*B.  The buffer form matters: an
@@ -300,36 +287,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E::kMi
                     constexpr int kGrp = E::kGrp;
                     T e[2][kGrp];
                     float fv[2][kGrp];
-                    // fixed-point form (EF16X/EBF16X): rows step by round(U 2^16) from the
-                    // run's middle voxel; the integer part is the entry's LDS index
-                    // (entry base + v0 >= 0 inside the box, the tile's start for inactive
-                    // lanes), the fraction is fv 2^16
-                    unsigned vfm = 0, Uf = 0;
-                    (void)vfm;
-                    (void)Uf;
-#if defined(__HIP_DEVICE_COMPILE__)
-                    if constexpr (E::kFix) {
-                        const unsigned lds_tile =
-                            (unsigned)(uintptr_t)(const __attribute__((address_space(3))) T*)tile;
-                        const float vmid = det_row_index((float)(ZPL / 2 + koff), Ue, vA);
-                        Uf = (unsigned)__builtin_rintf(Ue * 65536.0f);
-                        vfm = (unsigned)__builtin_rintf(vmid * 65536.0f) + ((lds_tile / (unsigned)E::kBytes + (unsigned)base) << 16);
-                    }
-#endif
                     auto fetch = [&](int g0, int slot) {
 #pragma unroll
                         for (int j = 0; j < kGrp; ++j) {
-                            if constexpr (E::kFix) {
-#if defined(__HIP_DEVICE_COMPILE__)
-                                const unsigned vf = vfm - (unsigned)(g0 + j - ZPL / 2) * Uf;
-                                e[slot][j] = *(const __attribute__((address_space(3))) T*)(uintptr_t)((vf >> 16) * (unsigned)E::kBytes);
-                                fv[slot][j] = (float)(vf & 0xffffu);
-#endif
-                            } else {
-                                const float vI = det_row_index((float)(g0 + j + koff), Ue, vA);
-                                e[slot][j] = tb[cvt_flr(vI)];
-                                fv[slot][j] = __builtin_amdgcn_fractf(vI);
-                            }
+                            const float vI = det_row_index((float)(g0 + j + koff), Ue, vA);
+                            e[slot][j] = tb[cvt_flr(vI)];
+                            fv[slot][j] = __builtin_amdgcn_fractf(vI);
                         }
                     };
                     fetch(0, 0);
@@ -353,7 +316,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E::kMi
                             const float vI = det_row_index((float)(k + koff), U, vA);
                             const int v0 = (int)vI;
                             const T e = tb[v0];
-                            acc[k] = fmaf(w, E::eval(e, fu, (vI - (float)v0) * (E::kFix ? 65536.0f : 1.0f)), acc[k]);
+                            acc[k] = fmaf(w, E::eval(e, fu, vI - (float)v0), acc[k]);
                         }
                     }
                 }
@@ -368,7 +331,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E::kMi
                         const float vI = det_row_index((float)(k + koff), U, vA);
                         const int v0 = (int)vI;
                         const T e = src[v0];
-                        acc[k] = fmaf(w, E::eval(e, fu, (vI - (float)v0) * (E::kFix ? 65536.0f : 1.0f)), acc[k]);
+                        acc[k] = fmaf(w, E::eval(e, fu, vI - (float)v0), acc[k]);
                     }
                 }
             }
@@ -383,7 +346,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E::kMi
             float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (pn < a.nproj) {
                 g = pp[pn];
-                g.w = (scales ? scales[pn] : 1.0f) * (E::kFix ? 0x1p-14f : 1.0f);
+                g.w = scales ? scales[pn] : 1.0f;
             }
             params[slot][tid] = g;
         };
@@ -661,13 +624,11 @@ ctg_status launch_zpl(const BpLaunch& L, BpArgs a, hipStream_t st) {
             CTG_BP_LAUNCH(EF32, 16);
             break;
         case CTG_STORE_F16:
-            if (layout_fixed_point()) CTG_BP_LAUNCH(EF16X, 4);
-            else if (a.dma16) CTG_BP_LAUNCH(EF16, 16);
+            if (a.dma16) CTG_BP_LAUNCH(EF16, 16);
             else CTG_BP_LAUNCH(EF16, 4);
             break;
         case CTG_STORE_BF16:
-            if (layout_fixed_point()) CTG_BP_LAUNCH(EBF16X, 4);
-            else if (a.dma16) CTG_BP_LAUNCH(EBF16, 16);
+            if (a.dma16) CTG_BP_LAUNCH(EBF16, 16);
             else CTG_BP_LAUNCH(EBF16, 4);
             break;
         default:

@@ -5,7 +5,6 @@
 
 #include <atomic>
 #include <cstdarg>
-#include <cstdlib>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -122,11 +121,6 @@ ctg_status fill_u32(void* d, uint32_t value, size_t count, hipStream_t st);
 ctg_status launch_maxabs(const float* d_images, size_t per_image, int nproj, unsigned* d_maxabs, hipStream_t st);
 
 ctg_status ensure_filter_tables(ctg_context* ctx, int W, int filter);
-
-// FP16/BF16 entries in the fixed-point-row form (CTG_BP_FIX16 set): (A, dU) scaled by
-// 2^14 and (dV, dUV) by 2^-2 in the layout, read with a 16.16 row index by the
-// backprojection.  Experimental switch, read at every launch of both.
-inline bool layout_fixed_point() { return std::getenv("CTG_BP_FIX16") != nullptr; }
 
 // ---- preprocess.hip
 ctg_status launch_preprocess_rows(ctg_context* ctx, const void* d_raw, ctg_pixel_type pixel,

@@ -520,7 +520,7 @@ __device__ inline uint32_t f2bf(float x) {
 }
 __device__ inline uint32_t pack_bf16(float a, float b) { return f2bf(a) | (f2bf(b) << 16); }
 
-template <int STORE, bool FIX = false>
+template <int STORE>
 __global__ __launch_bounds__(256) void layout_kernel(const float* __restrict__ rows, int W, int H,
                                                      const unsigned* __restrict__ maxabs,
                                                      void* __restrict__ out, float* __restrict__ scales) {
@@ -552,15 +552,13 @@ __global__ __launch_bounds__(256) void layout_kernel(const float* __restrict__ r
             const float A = tile[vl][ul] * inv, B = tile[vl][ul + 1] * inv;
             const float C = tile[vl + 1][ul] * inv, D = tile[vl + 1][ul + 1] * inv;
             const float dU = B - A, dV = C - A, dUV = (D - C) - dU;
-            // fixed-point-row form: (A, dU) times 2^14, (dV, dUV) times 2^-2 (exact)
-            const float hs = FIX ? 0x1p14f : 1.0f, ls = FIX ? 0x1p-2f : 1.0f;
             size_t idx = (p * (size_t)W + u) * H + v;
             if (STORE == CTG_STORE_F32) {
                 reinterpret_cast<float4*>(out)[idx] = make_float4(A, dU, dV, dUV);
             } else if (STORE == CTG_STORE_F16) {
-                reinterpret_cast<uint2*>(out)[idx] = make_uint2(pack_f16(A * hs, dU * hs), pack_f16(dV * ls, dUV * ls));
+                reinterpret_cast<uint2*>(out)[idx] = make_uint2(pack_f16(A, dU), pack_f16(dV, dUV));
             } else {
-                reinterpret_cast<uint2*>(out)[idx] = make_uint2(pack_bf16(A * hs, dU * hs), pack_bf16(dV * ls, dUV * ls));
+                reinterpret_cast<uint2*>(out)[idx] = make_uint2(pack_bf16(A, dU), pack_bf16(dV, dUV));
             }
         }
     }
@@ -750,19 +748,11 @@ ctg_status launch_layout(const ctg_geometry& g, const float* d_rows, int nproj, 
             hipLaunchKernelGGL((layout_kernel<CTG_STORE_F32>), grid, dim3(256), 0, st, rows, g.det_width,
                                g.det_height, mx, out, sc);
         } else if (store == CTG_STORE_F16) {
-            if (layout_fixed_point())
-                hipLaunchKernelGGL((layout_kernel<CTG_STORE_F16, true>), grid, dim3(256), 0, st, rows, g.det_width,
-                                   g.det_height, mx, out, sc);
-            else
-                hipLaunchKernelGGL((layout_kernel<CTG_STORE_F16>), grid, dim3(256), 0, st, rows, g.det_width,
-                                   g.det_height, mx, out, sc);
+            hipLaunchKernelGGL((layout_kernel<CTG_STORE_F16>), grid, dim3(256), 0, st, rows, g.det_width,
+                               g.det_height, mx, out, sc);
         } else {
-            if (layout_fixed_point())
-                hipLaunchKernelGGL((layout_kernel<CTG_STORE_BF16, true>), grid, dim3(256), 0, st, rows, g.det_width,
-                                   g.det_height, mx, out, sc);
-            else
-                hipLaunchKernelGGL((layout_kernel<CTG_STORE_BF16>), grid, dim3(256), 0, st, rows, g.det_width,
-                                   g.det_height, mx, out, sc);
+            hipLaunchKernelGGL((layout_kernel<CTG_STORE_BF16>), grid, dim3(256), 0, st, rows, g.det_width,
+                               g.det_height, mx, out, sc);
         }
         CTG_HIP(hipGetLastError());
     }
