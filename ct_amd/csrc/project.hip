@@ -419,6 +419,7 @@ struct FpWin {
     int wz0;     // first grid slice of the window (a multiple of 64 past the run base)
     int rbase;   // first entry row any voxel of the window can reach
     int span;    // rows per copy
+    int stride;  // entries between copies: span rounded up to 8 mod 16 (see the kernel)
 };
 
 __device__ inline int flr_i32(float x) {
@@ -434,13 +435,17 @@ __global__ __launch_bounds__(kGThreads) void fp_entry_kernel(FpArgs a, FpWin win
     __shared__ int4 slots[kESlots];      // column, U, w, w fu
     __shared__ int slot_z[kESlots];      // z interval relative to the run base
     __shared__ int wave_cnt[kGWaves];
-    extern __shared__ float4 acc_lds[];  // [wave][row][parity]
+    // [wave][parity][row], copies `stride` entries apart.  Rows fall by about U per lane,
+    // so the even lanes of a group touch rows r, r - 2, ... of one copy and the odd lanes
+    // rows r - 1, r - 3, ... of the next: with the copies 8 entries apart modulo 16 the
+    // 16-byte bank groups of neighbouring lanes interleave instead of colliding.
+    extern __shared__ float4 acc_lds[];
     const int c = blockIdx.x;
     const int p = blockIdx.y;
     const int tid = (int)threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int span = win.span;
-    for (int i = tid; i < 2 * kGWaves * span; i += kGThreads) acc_lds[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int span = win.span, stride = win.stride;
+    for (int i = tid; i < 2 * kGWaves * stride; i += kGThreads) acc_lds[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 
     const float4 g = pp[p];
     const float sn = g.x, cs = g.y, h = g.z;
@@ -484,8 +489,8 @@ __global__ __launch_bounds__(kGThreads) void fp_entry_kernel(FpArgs a, FpWin win
 #pragma unroll
     for (int k = 0; k < G; ++k) zh[k] = ((float)(win.wz0 + 64 * k + (lane & 32)) - a.zpre) + h;
     const float lane_k = (float)(lane & 31);
-    // LDS byte address of entry row v0 in this lane's copy: v0 * 32 + lane_base
-    const int lane_base = ((wv * span) * 2 + (lane & 1)) * 16 - win.rbase * 32;
+    // LDS byte address of entry row v0 in this lane's copy: v0 * 16 + lane_base
+    const int lane_base = (wv * 2 + (lane & 1)) * stride * 16 - win.rbase * 16;
     char* acc_bytes = reinterpret_cast<char*>(acc_lds);
     const float* colbase = vol + (zrb - a.z0);
     const int zc = win.wz0 - zrb;  // the window's first slice relative to the run base
@@ -615,7 +620,7 @@ __global__ __launch_bounds__(kGThreads) void fp_entry_kernel(FpArgs a, FpWin win
                     const int v0 = flr_i32(vI);
                     const float fv = __builtin_amdgcn_fractf(vI);
                     const float t = sl.it.w * sl.x[k], tu = sl.it.wfu * sl.x[k];
-                    float4* e = reinterpret_cast<float4*>(acc_bytes + (v0 * 32 + lane_base));
+                    float4* e = reinterpret_cast<float4*>(acc_bytes + (v0 * 16 + lane_base));
                     float4 v = *e;
                     v.x += t;
                     v.y += tu;
@@ -650,7 +655,7 @@ __global__ __launch_bounds__(kGThreads) void fp_entry_kernel(FpArgs a, FpWin win
         if (r >= 0 && r < span) {
             for (int w = 0; w < kGWaves; ++w)
                 for (int par = 0; par < 2; ++par) {
-                    const float4 e = acc_lds[(w * span + r) * 2 + par];
+                    const float4 e = acc_lds[(w * 2 + par) * stride + r];
                     sum.x += e.x; sum.y += e.y; sum.z += e.z; sum.w += e.w;
                 }
         }
@@ -897,7 +902,11 @@ ctg_status launch_forward_project(const ctg_geometry& g, const float4* d_params,
             FpWin win{};
             win.wz0 = w0;
             window_rows(g, hmin, hmax, Umin, Umax, w0, 64 * G, win.rbase, win.span);
-            const size_t lds = sizeof(float4) * 2 * kGWaves * (size_t)win.span;
+            // copies 8 entries apart modulo 16 (CTG_FP_SPAD=r overrides the remainder, tuning)
+            int pad = 8;
+            if (const char* e = std::getenv("CTG_FP_SPAD")) pad = std::atoi(e) & 15;
+            win.stride = win.span + ((pad - win.span) & 15);
+            const size_t lds = sizeof(float4) * 2 * kGWaves * (size_t)win.stride;
             if (G == 4) hipLaunchKernelGGL(fp_entry_kernel<4>, grid, dim3(kGThreads), lds, st, a, win, d_params, d_volume, ps);
             else if (G == 2) hipLaunchKernelGGL(fp_entry_kernel<2>, grid, dim3(kGThreads), lds, st, a, win, d_params, d_volume, ps);
             else hipLaunchKernelGGL(fp_entry_kernel<1>, grid, dim3(kGThreads), lds, st, a, win, d_params, d_volume, ps);
