@@ -224,6 +224,8 @@ def main():
     ap.add_argument("--slab", type=int, default=0, help="z-slab height (other workloads)")
     ap.add_argument("--checkpoint-dir", default="",
                     help="refine768: write per-slab checkpoints to this (empty) directory")
+    ap.add_argument("--checkpoint-every", type=int, default=1,
+                    help="refine768: checkpoint every this many steps (and the last)")
     ap.add_argument("--calib-copy-gib", type=float, default=0.0,
                     help="run a device-to-device copy of this size first (HBM counter calibration)")
     args = ap.parse_args()

@@ -236,10 +236,11 @@ def refine768(args):
     ckdir = getattr(args, "checkpoint_dir", "") or None
     if ckdir and os.path.exists(os.path.join(ckdir, "state.json")):
         raise SystemExit(f"{ckdir} holds a checkpoint of an earlier run; give an empty directory")
-    x, _ = refine(op, x, lineint, 10, checkpoint_dir=ckdir, on_step=on_step)
+    every = max(1, getattr(args, "checkpoint_every", 1) or 1)
+    x, _ = refine(op, x, lineint, 10, checkpoint_dir=ckdir, on_step=on_step, every=every)
     out = {"workload": f"sparse-view refinement: {n}^3 from {P}x{H}x{W}, Poisson I0=5e3, FP16 FDK seed, "
                        f"10 steepest-descent steps in z-slabs of {op.slab_nz}"
-                       + (", per-slab checkpoints" if ckdir else ""),
+                       + (f", per-slab checkpoints every {every} step(s)" if ckdir else ""),
            "fdk_scale_to_mu_per_voxel": scale, "fdk_seed_least_squares_scale": ls_scale,
            "rmse_ls_scaled_fdk_vs_truth": rmse(fdk["f16"] * ls_scale, truth),
            "rmse_fp32_fdk_vs_truth": rmse(fdk["f32"], truth), "rmse_fp16_fdk_vs_truth": rmse(fdk["f16"], truth),

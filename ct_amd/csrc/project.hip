@@ -490,7 +490,10 @@ __global__ __launch_bounds__(kGThreads) void fp_entry_kernel(FpArgs a, FpWin win
     for (int k = 0; k < G; ++k) zh[k] = ((float)(win.wz0 + 64 * k + (lane & 32)) - a.zpre) + h;
     const float lane_k = (float)(lane & 31);
     // LDS byte address of entry row v0 in this lane's copy: v0 * 16 + lane_base
-    const int lane_base = (wv * 2 + (lane & 1)) * stride * 16 - win.rbase * 16;
+    int lane_base = (wv * 2 + (lane & 1)) * stride * 16 - win.rbase * 16;
+    // opaque to the optimiser, so that v0 * 16 + lane_base stays one shift-and-add per
+    // voxel instead of being re-associated around rbase
+    asm volatile("" : "+v"(lane_base));
     char* acc_bytes = reinterpret_cast<char*>(acc_lds);
     const float* colbase = vol + (zrb - a.z0);
     const int zc = win.wz0 - zrb;  // the window's first slice relative to the run base
@@ -593,9 +596,12 @@ __global__ __launch_bounds__(kGThreads) void fp_entry_kernel(FpArgs a, FpWin win
                 const int hi1 = (zb >> kFpZBits) & ((1 << kFpZBits) - 1);  // one past the last slice
                 it.rs = make_vbuf(colbase + (size_t)__builtin_amdgcn_readfirstlane(r.x) * a.zstride + it.lo,
                                   valid ? (hi1 - it.lo) * (int)sizeof(float) : 0);
-                it.U = __int_as_float(__builtin_amdgcn_readfirstlane(r.y));
-                it.w = __int_as_float(__builtin_amdgcn_readfirstlane(r.z));
-                it.wfu = __int_as_float(__builtin_amdgcn_readfirstlane(r.w));
+                // the same value in every lane: kept in vector registers (operands of the
+                // per-voxel arithmetic), where moving them to scalars would cost a
+                // readfirstlane each
+                it.U = __int_as_float(r.y);
+                it.w = __int_as_float(r.z);
+                it.wfu = __int_as_float(r.w);
                 return it;
             };
             // Candidate ci goes to wave ci mod 4.  Three named slots keep the next two
@@ -879,12 +885,16 @@ ctg_status launch_forward_project(const ctg_geometry& g, const float4* d_params,
             hmax = std::max(hmax, (double)h_params[i].z);
         }
         const int zrb = z0 & ~(kRowAnchor - 1);
+        // CTG_FP_MAXSPAN overrides the rows per copy the window height may need (tuning;
+        // at most 400, which keeps the eight copies within 54 KB of LDS)
+        int max_span = kEMaxSpan;
+        if (const char* e = std::getenv("CTG_FP_MAXSPAN")) max_span = std::max(16, std::min(400, std::atoi(e)));
         for (int cand : {4, 2, 1}) {
             bool fits = true;
             for (int w0 = zrb; w0 < z0 + nz && fits; w0 += 64 * cand) {
                 int rb, sp;
                 window_rows(g, hmin, hmax, Umin, Umax, w0, 64 * cand, rb, sp);
-                fits = sp <= kEMaxSpan;
+                fits = sp <= max_span;
             }
             if (fits) {
                 G = cand;

@@ -13,16 +13,19 @@ written to disk together with the step state, so a run that stops (a crash, a ki
 
 * ``iterate.bin``: the committed iterate x_k, slab after slab (slab s is its
   ``(nx, ny, nz_s)`` block, contiguous), float32;
-* ``next.bin``: the slabs of x_{k+1} written so far, same layout;
-* ``state.json``: ``{"step": k, "slab": s, "alpha": ..., "loss": ..., "history": [...]}``,
-  replaced atomically (write + rename) after each slab; ``slab`` = -1 between steps.
+* ``next.bin``: the slabs written so far of the step being checkpointed, same layout;
+* ``state.json``: ``{"step": k, "slab": s, "target": t, "alpha": ..., "loss": ...,
+  "history": [...]}`` -- k the committed step, and while step t is being written its
+  last slab on disk and its line search (``slab`` = -1 between checkpoints) --
+  replaced atomically (write + rename) after each slab.
 
 When the last slab of a step is on disk, ``next.bin`` replaces ``iterate.bin`` (a
-rename) and the state moves to step k+1.  A resumed run reloads x_k, recomputes r and
-g (the projector and the backprojection are deterministic: no atomics, fixed
-summation order), takes alpha from the state, reloads the slabs of x_{k+1} already
-written and continues with the next slab; its result equals an uninterrupted run's
-bit for bit.
+rename) and the state moves to that step.  Steps can be checkpointed every few
+(``every``); the others write nothing.  A resumed run reloads the committed iterate,
+recomputes the steps after it (the projector and the backprojection are
+deterministic: no atomics, fixed summation order), and in a step stopped part-way
+takes alpha from the state, reloads the slabs already written and continues with the
+next slab; its result equals an uninterrupted run's bit for bit.
 """
 from __future__ import annotations
 
@@ -113,13 +116,15 @@ class SlabCheckpoint:
 
 
 def refine(op: ConeBeamOperator, x0: torch.Tensor, measured: torch.Tensor, steps: int,
-           checkpoint_dir: str | None = None, stop_after=None, on_step=None):
+           checkpoint_dir: str | None = None, stop_after=None, on_step=None, every: int = 1):
     """Runs (or resumes) ``steps`` steepest-descent steps from ``x0``.
 
     op: the operator (its ``slab_nz`` sets the checkpoint granularity).
     x0: the seed (used only when the checkpoint directory holds no state).
     measured: line integrals p, (P, H, W) float32 on the operator's device.
     checkpoint_dir: where the per-slab checkpoints go (None: no checkpoints).
+    every: checkpoint the steps whose number is a multiple of this (and the last); a
+        resumed run recomputes the steps after the last committed one.
     stop_after: (step, slab) after whose write the run raises :class:`Interrupted`
         (steps count from 1, slabs from 0), to exercise the resume path.
     on_step: called as on_step(record, x) after every step.
@@ -129,62 +134,82 @@ def refine(op: ConeBeamOperator, x0: torch.Tensor, measured: torch.Tensor, steps
     slabs = list(op._slabs())
     ck = SlabCheckpoint(checkpoint_dir, op.volume_shape, slabs) if checkpoint_dir else None
     x = torch.empty(op.volume_shape, dtype=torch.float32, device=op.device)
+    # timing waits for the device (an operator on host tensors, as in the CPU tests of
+    # the checkpoint logic, has nothing to wait for)
+    sync = torch.cuda.synchronize if x.is_cuda else (lambda: None)
+    if every < 1:
+        raise ValueError("every must be at least 1")
+    # state: "step" = the committed step (iterate.bin holds its iterate); while a
+    # checkpointed step "target" is being written, "slab" = its last slab on disk and
+    # "alpha"/"loss" its line search (otherwise slab = -1)
     st = ck.state() if ck else None
+    mid = None  # (target, slab, alpha, loss) of a step stopped part-way
     if st is None:
         x.copy_(x0)
-        st = {"step": 0, "slab": -1, "alpha": None, "loss": None, "history": []}
+        committed, history = 0, []
         if ck:
             ck.save_iterate(x)
-            ck.write_state(st)
+            ck.write_state({"step": 0, "slab": -1, "target": None, "alpha": None, "loss": None, "history": []})
     else:
-        if st["slab"] == len(slabs) - 1 and not os.path.exists(os.path.join(checkpoint_dir, "next.bin")):
-            # stopped between the commit's rename and its state write: the step is done
-            st = {"step": st["step"] + 1, "slab": -1, "alpha": None, "loss": None, "history": st["history"]}
+        committed = st["step"]
+        history = list(st["history"][:committed])
+        if st["slab"] >= 0:
+            if st["slab"] == len(slabs) - 1 and not os.path.exists(os.path.join(checkpoint_dir, "next.bin")):
+                # stopped between the commit's rename and its state write: the step is done
+                history = list(st["history"][:st["target"] - 1])
+                history.append({"step": st["target"], "loss_before": st["loss"], "alpha": st["alpha"], "ms": None,
+                                "ms_checkpoint": None})
+                committed = st["target"]
+            else:
+                mid = (st["target"], st["slab"], st["alpha"], st["loss"])
         ck.load_iterate(x)
-    history = list(st["history"])
-    while st["step"] < steps:
-        k = st["step"] + 1
-        torch.cuda.synchronize()
+    k = committed
+    while k < steps:
+        k += 1
+        ckpt = ck is not None and (k % every == 0 or k == steps)
+        resume = mid is not None and mid[0] == k
+        sync()
         t0 = time.perf_counter()
         r = op.project(x) - measured
         g = op.backproject(r)
-        if st["slab"] >= 0:
-            alpha, loss = st["alpha"], st["loss"]
+        if resume:
+            alpha, loss = mid[2], mid[3]
         else:
             ag = op.project(g)
             alpha = _sum_sq(g) / _sum_sq(ag)
             loss = 0.5 * _sum_sq(r)
             del ag
         del r
-        torch.cuda.synchronize()
+        sync()
         ms_grad = (time.perf_counter() - t0) * 1e3
         ms_ck = 0.0
-        first = st["slab"] + 1
-        if ck and first > 0:  # slabs of x_{k+1} written before the stop
+        first = mid[1] + 1 if resume else 0
+        if first > 0:  # slabs of this step written before the stop
             ck.load_iterate(x, name="next.bin", upto=first)
         t1 = time.perf_counter()
         for s in range(first, len(slabs)):
             z0, nz = slabs[s]
             x[:, :, z0:z0 + nz] -= alpha * g[:, :, z0:z0 + nz]
-            if ck:
+            if ckpt:
                 tc = time.perf_counter()
                 ck.write_next_slab(s, x[:, :, z0:z0 + nz])
-                st = {"step": k - 1, "slab": s, "alpha": alpha, "loss": loss, "history": history}
-                ck.write_state(st)
+                ck.write_state({"step": committed, "slab": s, "target": k, "alpha": alpha, "loss": loss,
+                                "history": history})
                 ms_ck += (time.perf_counter() - tc) * 1e3
             if stop_after is not None and (k, s) == tuple(stop_after):
                 raise Interrupted(f"stopped after slab {s} of step {k}")
-        torch.cuda.synchronize()
+        sync()
         ms_update = (time.perf_counter() - t1) * 1e3 - ms_ck
         del g
         rec = {"step": k, "loss_before": loss, "alpha": alpha, "ms": ms_grad + ms_update, "ms_checkpoint": ms_ck}
         history.append(rec)
-        st = {"step": k, "slab": -1, "alpha": None, "loss": None, "history": history}
-        if ck:
+        if ckpt:
             tc = time.perf_counter()
             ck.commit()
-            ck.write_state(st)
+            committed = k
+            ck.write_state({"step": k, "slab": -1, "target": None, "alpha": None, "loss": None, "history": history})
             rec["ms_checkpoint"] += (time.perf_counter() - tc) * 1e3
+        mid = None
         if on_step:
             on_step(rec, x)
     return x, history

@@ -111,3 +111,16 @@ def test_refinement_resumes_from_a_slab_checkpoint(gpu, tmp_path):
     # a run over a finished checkpoint returns its iterate without further steps
     again, _ = refine(op, x0, lineint, 3, checkpoint_dir=ck)
     assert torch.equal(again, ref)
+    # checkpoints every second step: a stop in a step that writes nothing (3) and one
+    # part-way through a checkpointed step (4) both resume from step 2's iterate
+    ref4, hist4 = refine(op, x0, lineint, 4)
+    for stop in ((3, 0), (4, 1)):
+        ck2 = str(tmp_path / f"ck_{stop[0]}")
+        with pytest.raises(Interrupted):
+            refine(op, x0, lineint, 4, checkpoint_dir=ck2, stop_after=stop, every=2)
+        state = json.load(open(os.path.join(ck2, "state.json")))
+        assert state["step"] == 2
+        got, hist = refine(op, x0, lineint, 4, checkpoint_dir=ck2, every=2)
+        torch.cuda.synchronize()
+        assert torch.equal(got, ref4), stop
+        assert [h["loss_before"] for h in hist] == [h["loss_before"] for h in hist4]
