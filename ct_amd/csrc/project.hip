@@ -412,7 +412,7 @@ __global__ __launch_bounds__(kGThreads) void fp_gather_kernel(FpArgs a, const fl
 //    after another on one stream; fp_finish_kernel then adds ps[p][c][r].x +
 //    ps[p][c-1][r].y into the row-major image.
 // The order of every addition is fixed: the result does not depend on scheduling.
-constexpr int kESlots = 512;
+constexpr int kESlots = 256;
 constexpr int kEMaxSpan = 256;  // entry rows per copy
 
 struct FpWin {
@@ -432,13 +432,15 @@ template <int G>
 __global__ __launch_bounds__(kGThreads) void fp_entry_kernel(FpArgs a, FpWin win, const float4* __restrict__ pp,
                                                              const float* __restrict__ vol,
                                                              float2* __restrict__ ps) {
-    __shared__ int4 slots[kESlots];      // column, U, w, w fu
-    __shared__ int slot_z[kESlots];      // z interval relative to the run base
+    // column, U, w fu and the z interval relative to the run base (w = U^2 is recomputed
+    // from U): one 16-byte LDS read per item
+    __shared__ int4 slots[kESlots];
     __shared__ int wave_cnt[kGWaves];
-    // [wave][parity][row], copies `stride` entries apart.  Rows fall by about U per lane,
-    // so the even lanes of a group touch rows r, r - 2, ... of one copy and the odd lanes
-    // rows r - 1, r - 3, ... of the next: with the copies 8 entries apart modulo 16 the
-    // 16-byte bank groups of neighbouring lanes interleave instead of colliding.
+    // [wave][parity][row], copies `stride` entries apart (a multiple of 16 entries unless
+    // the host is told otherwise).  On gfx950 a ds_write_b128 occupies the LDS for about
+    // 13.5 cycles per wave instruction whatever its lane mask and a conflict-free
+    // ds_read_b128 about 4 (16 lanes per cycle); of the copy offsets 0, 2, 5, 6, 8 and 13
+    // entries modulo 16, 0 was the fastest on the 512^3 x 360 workload.
     extern __shared__ float4 acc_lds[];
     const int c = blockIdx.x;
     const int p = blockIdx.y;
@@ -572,9 +574,9 @@ __global__ __launch_bounds__(kGThreads) void fp_entry_kernel(FpArgs a, FpWin win
                     hi = min(hi, whi);
                 }
                 const float w = U * U;
-                slots[ci] = make_int4(r.x, r.y, __float_as_int(w), __float_as_int(w * fu));
                 // an empty interval [lo, hi] with hi < lo becomes a zero-length resource
-                slot_z[ci] = (lo - zrb) | (max(hi, lo - 1) - zrb + 1) << kFpZBits;
+                slots[ci] = make_int4(r.x, r.y, __float_as_int(w * fu),
+                                      (lo - zrb) | (max(hi, lo - 1) - zrb + 1) << kFpZBits);
             }
             ncand += total;
             __syncthreads();  // records complete; wave_cnt free again
@@ -590,7 +592,7 @@ __global__ __launch_bounds__(kGThreads) void fp_entry_kernel(FpArgs a, FpWin win
             // resource (they return zero without touching memory)
             auto item_of = [&](int ci, bool valid) {
                 const int4 r = slots[valid ? ci : 0];
-                const int zb = __builtin_amdgcn_readfirstlane(slot_z[valid ? ci : 0]);
+                const int zb = __builtin_amdgcn_readfirstlane(r.w);
                 Item it;
                 it.lo = zb & ((1 << kFpZBits) - 1);
                 const int hi1 = (zb >> kFpZBits) & ((1 << kFpZBits) - 1);  // one past the last slice
@@ -600,8 +602,8 @@ __global__ __launch_bounds__(kGThreads) void fp_entry_kernel(FpArgs a, FpWin win
                 // per-voxel arithmetic), where moving them to scalars would cost a
                 // readfirstlane each
                 it.U = __int_as_float(r.y);
-                it.w = __int_as_float(r.z);
-                it.wfu = __int_as_float(r.w);
+                it.w = it.U * it.U;  // as in the fill: the weight is U^2
+                it.wfu = __int_as_float(r.z);
                 return it;
             };
             // Candidate ci goes to wave ci mod 4.  Three named slots keep the next two
@@ -618,21 +620,31 @@ __global__ __launch_bounds__(kGThreads) void fp_entry_kernel(FpArgs a, FpWin win
                 __builtin_amdgcn_sched_barrier(0);  // keep the slots' loads in walk order
                 return valid;
             };
+            // The G chunks of one item touch distinct entries of each copy (same-parity
+            // lanes of the item are two or more slices apart, so their rows differ), so
+            // all G reads are issued before the first write: one LDS round trip per item
+            // instead of one per chunk.
             auto process = [&](const Slot& sl) {
+                float4* e[G];
+                float4 v[G];
+                float fv[G];
 #pragma unroll
                 for (int k = 0; k < G; ++k) {
                     const float vA = fmaf(-zh[k], sl.it.U, a.vpre);  // det_row_anchor(zwa, h, U, vpre)
                     const float vI = det_row_index(lane_k, sl.it.U, vA);
                     const int v0 = flr_i32(vI);
-                    const float fv = __builtin_amdgcn_fractf(vI);
+                    fv[k] = __builtin_amdgcn_fractf(vI);
+                    e[k] = reinterpret_cast<float4*>(acc_bytes + (v0 * 16 + lane_base));
+                    v[k] = *e[k];
+                }
+#pragma unroll
+                for (int k = 0; k < G; ++k) {
                     const float t = sl.it.w * sl.x[k], tu = sl.it.wfu * sl.x[k];
-                    float4* e = reinterpret_cast<float4*>(acc_bytes + (v0 * 16 + lane_base));
-                    float4 v = *e;
-                    v.x += t;
-                    v.y += tu;
-                    v.z = fmaf(t, fv, v.z);
-                    v.w = fmaf(tu, fv, v.w);
-                    *e = v;
+                    v[k].x += t;
+                    v[k].y += tu;
+                    v[k].z = fmaf(t, fv[k], v[k].z);
+                    v[k].w = fmaf(tu, fv[k], v[k].w);
+                    *e[k] = v[k];
                 }
             };
             Slot sa, sb, sc;
@@ -912,8 +924,9 @@ ctg_status launch_forward_project(const ctg_geometry& g, const float4* d_params,
             FpWin win{};
             win.wz0 = w0;
             window_rows(g, hmin, hmax, Umin, Umax, w0, 64 * G, win.rbase, win.span);
-            // copies 8 entries apart modulo 16 (CTG_FP_SPAD=r overrides the remainder, tuning)
-            int pad = 8;
+            // copies a multiple of 16 entries apart (CTG_FP_SPAD=r sets the remainder
+            // modulo 16 instead, tuning)
+            int pad = 0;
             if (const char* e = std::getenv("CTG_FP_SPAD")) pad = std::atoi(e) & 15;
             win.stride = win.span + ((pad - win.span) & 15);
             const size_t lds = sizeof(float4) * 2 * kGWaves * (size_t)win.stride;
