@@ -38,8 +38,10 @@ VALU_PEAK_T = 256 * 64 * 2.4e9 / 1e12
 # for the detector row, floor and fraction of it, three FMAs for the bilinear form and
 # one FMA into the accumulator (the LDS address is not counted)
 VALU_PER_UPDATE = 7.0
-# LDS bandwidth: 128 bytes per clock per CU, 256 CUs at 2.4 GHz, in TB/s
-LDS_PEAK_TBS = 128 * 256 * 2.4e9 / 1e12
+# LDS read bandwidth: 256 bytes per clock per CU (a ds_read_b128 serves 16 lanes per
+# 4-cycle pass: ct_amd/lds_rates measures at most 6.6 cycles per conflict-free wave
+# read and 16.9 for a 4-way conflict), 256 CUs at 2.4 GHz, in TB/s
+LDS_PEAK_TBS = 256 * 256 * 2.4e9 / 1e12
 
 
 def log(msg):
@@ -473,7 +475,7 @@ def main():
                      "traffic_over_compulsory": traffic / compulsory if traffic else None,
                      "valu_per_update": VALU_PER_UPDATE},
         # the other on-chip ceiling: one 16-byte FP32 entry read from LDS per update,
-        # against 128 B per clock per CU (256 CUs at 2.4 GHz)
+        # against 256 B per clock per CU (256 CUs at 2.4 GHz)
         "roofline_lds": {"bound": "lds", "achieved": 16.0 * updates / bp32 / 1e12, "peak": LDS_PEAK_TBS,
                          "unit": "TB/s", "frac": 16.0 * updates / bp32 / 1e12 / LDS_PEAK_TBS, "bytes_per_update": 16},
         # the 8-byte read-modify-write model of the metric's definition (the reference
