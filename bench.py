@@ -219,7 +219,7 @@ def main():
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end run from pinned host memory")
     ap.add_argument("--no-roi", action="store_true", help="skip the region-of-interest run")
     ap.add_argument("--workload", default="fdk1024",
-                    choices=["fdk1024", "precision2048", "gradient512", "refine768", "rowfilter1024"],
+                    choices=["fdk1024", "precision2048", "gradient512", "refine768", "rowfilter1024", "files1024"],
                     help="fdk1024: the headline line below; the others: bench_workloads.py")
     ap.add_argument("--det-w", type=int, default=0, help="detector width (other workloads)")
     ap.add_argument("--det-h", type=int, default=0, help="detector height (other workloads)")

@@ -17,6 +17,8 @@ refine768      sparse view: 120 projections of 1024x768 (FCD 5000 px), 768^3 vol
                --checkpoint-dir every finished slab of each step is checkpointed there);
                RMSE against the voxelised analytic phantom and against the plain FP32
                FDK, ms per step.
+files1024      the CtVolume facade (the path of ct_cli) from 720 PGM files of 1024^2: load,
+               reconstruct and save, each timed, on CtVolume's 1002 x 1002 x 1024 grid.
 rowfilter1024  the ramp-family row filter over 720 x 1024 rows of 1024 (Shepp-Logan):
                ctg_filter_rows (the register FFT) against the same circular filter as a
                dense circulant GEMM on the matrix cores (torch.matmul with FP32 inputs,
@@ -311,5 +313,108 @@ def rowfilter1024(args):
     ctx.close()
 
 
+def _write_scan_files(directory, host_raw, angles, pixel_mm, fcd_px, base):
+    """Ct's input format (Ct/CtVolume.cpp:136-330): a CSV of six parameter lines, one
+    ignored line, then `file<TAB>angle` per projection, and one 16-bit PGM per image
+    (big-endian samples).  Returns the CSV path and the bytes written."""
+    import numpy as np
+    img_dir = os.path.join(directory, "images")
+    os.makedirs(img_dir, exist_ok=True)
+    P, H, W = host_raw.shape
+    lines = ["images\tprojection directory", f"{pixel_mm!r}\tpixel size [mm]", "ccw\trotation direction",
+             "0.0\tu offset [mm]", f"{fcd_px * pixel_mm!r}\tFCD [mm]", f"{base!r}\tbase intensity", "ignored"]
+    header = f"P5\n{W} {H}\n65535\n".encode()
+    nbytes = 0
+    for p in range(P):
+        name = f"proj_{p:04d}.pgm"
+        with open(os.path.join(img_dir, name), "wb") as f:
+            f.write(header)
+            f.write(np.ascontiguousarray(host_raw[p], dtype=">u2").tobytes())
+        nbytes += len(header) + 2 * W * H
+        lines.append(f"{name}\t{float(angles[p])!r}")
+    csv = os.path.join(directory, "scan.csv")
+    with open(csv, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return csv, nbytes
+
+
+def files1024(args):
+    """The CtVolume facade (libctvolume.so, the path of ct_cli) from image files: the
+    default workload's 720 uint16 projections of 1024^2 written as PGM files with a CSV
+    (0.2 mm pixels, FCD 2500 px), then sinogramFromImages, reconstructVolume (Shepp-Logan
+    filter, GPU) and saveVolumeToBinaryFile (Z fastest, little endian) on CtVolume's own
+    1002 x 1002 x 1024 grid, each timed.  The files are read through the page cache
+    (they were just written), so the load and decode figures are CPU-bound decode rates,
+    not disk rates."""
+    import argparse
+    import ctypes
+    import shutil
+    import tempfile
+    import numpy as np
+    import torch
+    import ct_amd
+    import bench
+    P = args.nproj or 720
+    W = args.det_w or 1024
+    ctx = ct_amd.Context(0)
+    g, raw, angles, base = bench.build_workload(ctx, torch, argparse.Namespace(det=W, nproj=P, grid=1024))
+    host = raw.cpu().numpy().view(np.uint16)
+    del raw
+    ctx.close()
+    torch.cuda.empty_cache()
+    scratch = tempfile.mkdtemp(prefix="ct_files_")
+    try:
+        t = time.perf_counter()
+        csv, in_bytes = _write_scan_files(scratch, host, angles, 0.2, float(g.fcd), base)
+        log(f"wrote {P} PGM files ({in_bytes / 1e9:.2f} GB) in {time.perf_counter() - t:.1f} s")
+        lib = ctypes.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "ct_amd", "libctvolume.so"))
+        lib.ctv_create.restype = ctypes.c_void_p
+        for name in ("ctv_destroy", "ctv_reconstruct"):
+            getattr(lib, name).argtypes = [ctypes.c_void_p]
+        lib.ctv_load.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        lib.ctv_set_filter.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib.ctv_set_use_cuda.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib.ctv_save.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+        lib.ctv_sizes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_ulonglong)]
+        lib.ctv_status.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
+        h = lib.ctv_create()
+
+        def checked(what):
+            msg = ctypes.create_string_buffer(512)
+            if lib.ctv_status(h, msg, 512) != 0:  # 0 success, 1 interrupted, 2 error
+                raise RuntimeError(f"{what}: {msg.value.decode()}")
+
+        t0 = time.perf_counter()
+        if not lib.ctv_load(h, csv.encode()):
+            checked("sinogramFromImages")
+        t1 = time.perf_counter()
+        lib.ctv_set_filter(h, 1)  # shepplogan
+        lib.ctv_set_use_cuda(h, 1)
+        lib.ctv_reconstruct(h)
+        t2 = time.perf_counter()
+        checked("reconstructVolume")
+        out_path = os.path.join(scratch, "vol.raw")
+        lib.ctv_save(h, out_path.encode(), 0, 0)  # Z fastest, little endian
+        t3 = time.perf_counter()
+        checked("saveVolumeToBinaryFile")
+        sizes = (ctypes.c_ulonglong * 7)()
+        lib.ctv_sizes(h, sizes)
+        lib.ctv_destroy(h)
+        nx, ny, nz = int(sizes[0]), int(sizes[1]), int(sizes[2])
+        updates = float(nx) * ny * nz * P
+        out = {"workload": f"CtVolume facade from files: {P} PGM images of {W}x{W} (16-bit), shepplogan, "
+                           f"CtVolume grid {nx}x{ny}x{nz}, raw volume Z fastest LE",
+               "input_bytes": in_bytes, "volume_bytes": 4 * nx * ny * nz, "updates": updates,
+               "ms_load": (t1 - t0) * 1e3, "ms_reconstruct": (t2 - t1) * 1e3, "ms_save": (t3 - t2) * 1e3,
+               "ms_total": (t3 - t0) * 1e3, "gups_reconstruct": updates / (t2 - t1) / 1e9,
+               "gups_total": updates / (t3 - t0) / 1e9,
+               "load_gbs": in_bytes / (t1 - t0) / 1e9, "save_gbs": 4.0 * nx * ny * nz / (t3 - t2) / 1e9}
+        log(f"load {out['ms_load']:.0f} ms, reconstruct {out['ms_reconstruct']:.0f} ms "
+            f"({out['gups_reconstruct']:.0f} GUPS), save {out['ms_save']:.0f} ms")
+        print(json.dumps(out), flush=True)
+    finally:
+        shutil.rmtree(scratch, ignore_errors=True)
+
+
 WORKLOADS = {"precision2048": precision2048, "gradient512": gradient512, "refine768": refine768,
-             "rowfilter1024": rowfilter1024}
+             "rowfilter1024": rowfilter1024, "files1024": files1024}

@@ -116,8 +116,10 @@ std::string fmt_f3(double v) {
 
 double pixel_scale(PixelDepth d) { return d == PixelDepth::U8 ? 255.0 : (d == PixelDepth::U16 ? 65535.0 : 1.0); }
 
-// Decodes images[i] for i in idx with a pool of threads.
-void decode_all(const std::vector<std::string>& paths, std::vector<Image>& imgs, std::vector<std::string>& errs) {
+// Decodes paths[i] into imgs[i] with a pool of threads; on_image(i, image), when given,
+// runs on the decoding thread after each successful decode.
+void decode_all(const std::vector<std::string>& paths, std::vector<Image>& imgs, std::vector<std::string>& errs,
+                const std::function<void(size_t, const Image&)>& on_image = nullptr) {
     const size_t n = paths.size();
     imgs.assign(n, Image());
     errs.assign(n, std::string());
@@ -128,10 +130,60 @@ void decode_all(const std::vector<std::string>& paths, std::vector<Image>& imgs,
         pool.emplace_back([&]() {
             for (size_t i = next++; i < n; i = next++) {
                 if (!readImage(paths[i], imgs[i], errs[i])) imgs[i] = Image();
+                else if (on_image) on_image(i, imgs[i]);
             }
         });
     }
     for (auto& th : pool) th.join();
+}
+
+// Runs fn(begin, end) on parts of [0, n) of at least min_part elements, on up to 16
+// threads (the calling thread takes the first part).
+template <class F>
+void parallel_parts(size_t n, size_t min_part, F&& fn) {
+    const size_t hw = std::max(1u, std::min<unsigned>(16, std::thread::hardware_concurrency()));
+    const size_t parts = std::max<size_t>(1, std::min<size_t>(hw, n / std::max<size_t>(1, min_part)));
+    const size_t step = (n + parts - 1) / parts;
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < parts; ++t) {
+        const size_t b = t * step, e = std::min(n, b + step);
+        if (b < e) pool.emplace_back([&fn, b, e]() { fn(b, e); });
+    }
+    fn(0, std::min(n, step));
+    for (auto& th : pool) th.join();
+}
+
+// MemAvailable of /proc/meminfo in bytes (0 when it cannot be read).
+size_t host_memory_available() {
+    std::ifstream f("/proc/meminfo");
+    std::string line;
+    while (std::getline(f, line)) {
+        unsigned long long kb = 0;
+        if (std::sscanf(line.c_str(), "MemAvailable: %llu kB", &kb) == 1) return (size_t)kb * 1024;
+    }
+    return 0;
+}
+
+// the load keeps its decoded images for the reconstruction unless CTG_FACADE_IMAGE_CACHE=0
+bool image_cache_enabled() {
+    const char* e = std::getenv("CTG_FACADE_IMAGE_CACHE");
+    return !(e && std::string(e) == "0");
+}
+
+// Per-image minimum and maximum as Ct/CtVolume.cpp:271-281 scale them (by 256 for 8-bit
+// and 65536 for 16-bit and float images).
+std::pair<double, double> image_min_max(const Image& im) {
+    double lMin = std::numeric_limits<double>::infinity(), lMax = -lMin;
+    const size_t npx = (size_t)im.width * im.height;
+    for (size_t i = 0; i < npx; ++i) {
+        const double v = im.depth == PixelDepth::U8    ? im.data[i]
+                         : im.depth == PixelDepth::U16 ? reinterpret_cast<const uint16_t*>(im.data.data())[i]
+                                                       : reinterpret_cast<const float*>(im.data.data())[i];
+        lMin = std::min(lMin, v);
+        lMax = std::max(lMax, v);
+    }
+    const double div = im.depth == PixelDepth::U8 ? 256.0 : 65536.0;
+    return {lMin / div, lMax / div};
 }
 
 }  // namespace
@@ -379,6 +431,8 @@ bool CtVolume::sinogramFromImages(const std::string& csvFile) {
     stopActiveProcess_ = false;
     volume_.clear();
     sinogram_.clear();
+    imageCache_.reset();
+    imageCacheBytes_ = 0;
     std::ifstream in(csvFile);
     if (!in) {
         std::cerr << "Could not open CSV file - terminating" << std::endl;
@@ -476,6 +530,27 @@ bool CtVolume::sinogramFromImages(const std::string& csvFile) {
         unsigned rows = 0, cols = 0;
         double mn = std::numeric_limits<double>::quiet_NaN(), mx = std::numeric_limits<double>::quiet_NaN();
         mixedDepths_ = false;
+        // The decoded images are kept for the reconstruction when they all share the
+        // first image's size and depth and fit the budget (decided from the first image).
+        std::unique_ptr<uint8_t[]> cache;
+        size_t cacheStride = 0;
+        PixelDepth cacheDepth = PixelDepth::U16;
+        if (image_cache_enabled() && !paths.empty()) {
+            Image first;
+            std::string e;
+            if (readImage(paths[0], first, e) && !first.empty()) {
+                const size_t bytes = first.data.size() * paths.size();
+                if (bytes <= host_memory_available() / 3) {
+                    cache.reset(new (std::nothrow) uint8_t[bytes]);
+                    if (cache) {
+                        cacheStride = first.data.size();
+                        cacheDepth = first.depth;
+                    }
+                }
+            }
+        }
+        std::atomic<bool> cacheOk{cache != nullptr};
+        std::vector<std::pair<double, double>> minMax(chunk);
         for (size_t c0 = 0; c0 < paths.size(); c0 += chunk) {
             if (stopActiveProcess_) {
                 sinogram_.clear();
@@ -484,7 +559,16 @@ bool CtVolume::sinogramFromImages(const std::string& csvFile) {
                 return false;
             }
             std::vector<std::string> part(paths.begin() + c0, paths.begin() + std::min(paths.size(), c0 + chunk));
-            decode_all(part, imgs, errs);
+            // on the decoding threads: the image's min/max, and its copy into the cache
+            decode_all(part, imgs, errs, [&](size_t k, const Image& im) {
+                minMax[k] = image_min_max(im);
+                if (cacheOk.load(std::memory_order_relaxed)) {
+                    if (im.depth == cacheDepth && im.data.size() == cacheStride)
+                        std::memcpy(cache.get() + (c0 + k) * cacheStride, im.data.data(), cacheStride);
+                    else
+                        cacheOk = false;
+                }
+            });
             for (size_t k = 0; k < part.size(); ++k) {
                 const size_t cnt = c0 + k;
                 const Image& im = imgs[k];
@@ -510,19 +594,9 @@ bool CtVolume::sinogramFromImages(const std::string& csvFile) {
                     return false;
                 }
                 if (im.depth != imageDepth_) mixedDepths_ = true;
-                // per-image min/max scaled as in Ct/CtVolume.cpp:271-281
-                double lMin = std::numeric_limits<double>::infinity(), lMax = -lMin;
-                const size_t npx = (size_t)im.width * im.height;
-                for (size_t i = 0; i < npx; ++i) {
-                    double v = im.depth == PixelDepth::U8 ? im.data[i]
-                               : im.depth == PixelDepth::U16 ? reinterpret_cast<const uint16_t*>(im.data.data())[i]
-                                                             : reinterpret_cast<const float*>(im.data.data())[i];
-                    lMin = std::min(lMin, v);
-                    lMax = std::max(lMax, v);
-                }
-                const double div = im.depth == PixelDepth::U8 ? 256.0 : 65536.0;
-                lMin /= div;
-                lMax /= div;
+                // per-image min/max scaled as in Ct/CtVolume.cpp:271-281 (computed on the
+                // decoding thread), combined in image order
+                const double lMin = minMax[k].first, lMax = minMax[k].second;
                 if (std::isnan(mn) || lMin < mn) mn = lMin;
                 if (std::isnan(mx) || lMax > mx) mx = lMax;
                 imageWidth_ = cols;
@@ -541,6 +615,10 @@ bool CtVolume::sinogramFromImages(const std::string& csvFile) {
             return false;
         }
         std::cout << std::endl;
+        if (cacheOk && !mixedDepths_) {
+            imageCache_ = std::move(cache);
+            imageCacheBytes_ = cacheStride * paths.size();
+        }
     }
     makeHeightOffsetRelative();
     correctAngleDirection(rotationDirection);
@@ -570,12 +648,18 @@ void CtVolume::reconstructVolume() {
     }
     volume_.clear();
     try {
-        volume_.assign((size_t)xMax_ * yMax_ * zMax_, 0.0f);
+        volume_.resize((size_t)xMax_ * yMax_ * zMax_);
     } catch (...) {
         const std::string m = "The memory allocation for the volume failed. Maybe there is not enought free RAM.";
         std::cout << m << std::endl;
         finishReconstruction(CompletionStatus::error(m));
         return;
+    }
+    // zeroed on worker threads (the slabs not yet reconstructed read as 0 in the live
+    // cross-section); the first touch of the pages is spread over the threads too
+    {
+        float* v = volume_.data();
+        parallel_parts(volume_.size(), (size_t)1 << 22, [v](size_t b, size_t e) { std::fill(v + b, v + e, 0.0f); });
     }
     const auto start = std::chrono::steady_clock::now();
     auto fail = [&](const std::string& m) {
@@ -602,14 +686,30 @@ void CtVolume::reconstructVolume() {
     const size_t npx = (size_t)imageWidth_ * imageHeight_;
     const PixelDepth depth = mixedDepths_ ? PixelDepth::F32 : imageDepth_;
     const size_t bpp = depth == PixelDepth::U8 ? 1 : (depth == PixelDepth::U16 ? 2 : 4);
-    std::vector<uint8_t> raw;
-    try {
-        raw.resize(used.size() * npx * bpp);
-    } catch (...) {
-        fail("The memory allocation for the projections failed. Maybe there is not enought free RAM.");
-        return;
+    // The images: from the load's cache when it holds this sinogram (every image, one
+    // depth), gathered when only every projectionStep-th is used; otherwise decoded here.
+    std::vector<uint8_t, NoInitAllocator<uint8_t>> raw;
+    const uint8_t* images = nullptr;
+    const bool cached = imageCacheBytes_ > 0 && !mixedDepths_ && imageCacheBytes_ == sinogram_.size() * npx * bpp;
+    if (!cached || used.size() != sinogram_.size()) {
+        try {
+            raw.resize(used.size() * npx * bpp);
+        } catch (...) {
+            fail("The memory allocation for the projections failed. Maybe there is not enought free RAM.");
+            return;
+        }
     }
-    {
+    if (cached && used.size() == sinogram_.size()) {
+        images = imageCache_.get();
+    } else if (cached) {
+        const uint8_t* src = imageCache_.get();
+        uint8_t* dst = raw.data();
+        const size_t ib = npx * bpp;
+        parallel_parts(used.size(), 1, [&](size_t b, size_t e) {
+            for (size_t k = b; k < e; ++k) std::memcpy(dst + k * ib, src + used[k] * ib, ib);
+        });
+        images = raw.data();
+    } else {
         const size_t chunk = 64;
         std::vector<Image> imgs;
         std::vector<std::string> errs;
@@ -656,6 +756,7 @@ void CtVolume::reconstructVolume() {
                 }
             }
         }
+        images = raw.data();
     }
     std::vector<double> angles, heights;
     for (size_t p : used) {
@@ -701,7 +802,7 @@ void CtVolume::reconstructVolume() {
                                                                           : CTG_FILTER_HANN;
         // preprocessing always runs on the GPU; the CPU-preprocessing switch selects the
         // CPU path's log form (Ct/CtVolume.cpp:802-812 vs 892-904)
-        st = ctg_reconstruct(ctx, raw.data(), pix, (int32_t)used.size(), baseIntensity_,
+        st = ctg_reconstruct(ctx, images, pix, (int32_t)used.size(), baseIntensity_,
                              useCpuPreprocessing_ ? CTG_LOG_CPU_FORM : CTG_LOG_GPU_FORM, filt, CTG_STORE_F32,
                              volume_.data(), CTG_Z_FASTEST, progress, &cb);
     }
@@ -849,9 +950,16 @@ void CtVolume::saveVolumeToBinaryFile(const std::string& filename, IndexOrder in
     }
     float vmin = 0.0f, vmax = 0.0f;
     if (!volume_.empty()) {
-        auto mm = std::minmax_element(volume_.begin(), volume_.end());
-        vmin = *mm.first;
-        vmax = *mm.second;
+        // the volume's range for the .vgi, over parts of it on worker threads
+        const float* v = volume_.data();
+        std::mutex mm_lock;
+        vmin = vmax = v[0];
+        parallel_parts(volume_.size(), (size_t)1 << 22, [&](size_t b, size_t e) {
+            auto mm = std::minmax_element(v + b, v + e);
+            std::lock_guard<std::mutex> g(mm_lock);
+            vmin = std::min(vmin, *mm.first);
+            vmax = std::max(vmax, *mm.second);
+        });
     }
     {
         std::ofstream out(join(dir, baseName + ".vgi"), std::ios::binary | std::ios::trunc);

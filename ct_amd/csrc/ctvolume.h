@@ -9,8 +9,10 @@
 #include <cstdint>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "imageio.h"
@@ -18,6 +20,29 @@
 struct ctg_context;
 
 namespace ct {
+
+// An allocator whose value-initialisation leaves trivial elements uninitialised:
+// resize() of a multi-GB volume then costs no single-threaded zero fill (the facade
+// zeroes the volume on worker threads instead).
+template <class T>
+struct NoInitAllocator : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAllocator<U>;
+    };
+    NoInitAllocator() = default;
+    template <class U>
+    NoInitAllocator(const NoInitAllocator<U>&) noexcept {}
+    template <class U>
+    void construct(U* p) noexcept {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <class U, class... A>
+    void construct(U* p, A&&... a) {
+        ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+    }
+};
+using VolumeData = std::vector<float, NoInitAllocator<float>>;
 
 // Ct/CtVolume.h:50-54
 enum class FilterType { RAMLAK, SHEPP_LOGAN, HANN };
@@ -134,7 +159,7 @@ public:
     void onSavingFinished(FinishedFn f) { savingFinished_ = std::move(f); }
 
     // the reconstructed volume, Z_FASTEST [x][y][z] (empty before a reconstruction)
-    const std::vector<float>& volumeData() const { return volume_; }
+    const VolumeData& volumeData() const { return volume_; }
     std::string lastErrorMessage() const { return lastErrorMessage_; }
     CompletionStatus lastStatus() const { return lastStatus_; }
 
@@ -171,7 +196,13 @@ private:
     mutable std::atomic<bool> stopActiveProcess_{false};
     float xFrom_float_ = 0, xTo_float_ = 1, yFrom_float_ = 0, yTo_float_ = 1, zFrom_float_ = 0, zTo_float_ = 1;
 
-    std::vector<float> volume_;
+    VolumeData volume_;
+    // The images the last successful load decoded, in sinogram order and native byte
+    // order, kept when they share one depth and fit a third of the host's available
+    // memory (imageCacheBytes_ == 0 otherwise, or with CTG_FACADE_IMAGE_CACHE=0): the
+    // reconstruction then takes them instead of decoding every file a second time.
+    std::unique_ptr<uint8_t[]> imageCache_;
+    size_t imageCacheBytes_ = 0;
     unsigned xFrom_ = 0, xTo_ = 0, yFrom_ = 0, yTo_ = 0, zFrom_ = 0, zTo_ = 0;
     unsigned xMax_ = 0, yMax_ = 0, zMax_ = 0;
     std::pair<float, float> minMaxValues_{0.0f, 0.0f};

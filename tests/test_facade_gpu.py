@@ -122,3 +122,42 @@ def test_progress_reports_carry_the_cross_section(gpu, tmp_path):
         filled_prev = filled
     assert np.count_nonzero(np.any(reports[0][1] != 0, axis=1)) < np.count_nonzero(np.any(final != 0, axis=1))
     lib.ctv_on_cross_section_progress(f.h, CS_FN(), None)
+
+
+def test_images_kept_from_the_load_give_the_same_volume(gpu, tmp_path, monkeypatch):
+    """The load keeps its decoded images and the reconstruction takes them (gathered when
+    setSkipProjections(1) uses every second projection); with CTG_FACADE_IMAGE_CACHE=0
+    every file is decoded again at the reconstruction.  Both give the same volume bit
+    for bit, and a second load of another scan replaces the kept images."""
+    from scan_io import write_scan
+    lib = load_ctv()
+
+    def run(csv, skip):
+        f = Facade(lib)
+        assert f.load(csv)
+        lib.ctv_set_filter(f.h, 1)
+        lib.ctv_set_skip_projections(f.h, skip)
+        lib.ctv_reconstruct(f.h)
+        code, msg = f.status()
+        assert code == 0, msg
+        return f.volume().copy()
+
+    scans = []
+    for i, seed in enumerate((51, 52)):
+        scan = small_scan(W=64, H=48, P=30, seed=seed)
+        d = str(tmp_path / f"s{i}")
+        scans.append(write_scan(d, scan["raw"], scan["angles"], pixel_size=0.5, fcd_px=scan["fcd"]))
+    for skip in (0, 1):  # every projection, every second one
+        kept = run(scans[0], skip)
+        monkeypatch.setenv("CTG_FACADE_IMAGE_CACHE", "0")
+        decoded = run(scans[0], skip)
+        monkeypatch.delenv("CTG_FACADE_IMAGE_CACHE")
+        assert np.isfinite(kept).all() and np.abs(kept).max() > 0
+        assert np.array_equal(kept, decoded), skip
+    # one facade, two loads: the second scan's images replace the first's
+    f = Facade(lib)
+    assert f.load(scans[0]) and f.load(scans[1])
+    lib.ctv_set_filter(f.h, 1)
+    lib.ctv_reconstruct(f.h)
+    assert f.status()[0] == 0
+    assert np.array_equal(f.volume(), run(scans[1], 0))
