@@ -488,7 +488,15 @@ def main():
     if "f16" in res:
         t16, bp16 = res["f16"]
         out.update({"ms_per_recon_f16": t16 * 1e3, "bp_ms_f16": bp16 * 1e3, "bp_gups_f16": updates / bp16 / 1e9,
-                    "f16_over_f32_bp": bp32 / bp16, "f16_over_f32_step": t32 / t16})
+                    "f16_over_f32_bp": bp32 / bp16, "f16_over_f32_step": t32 / t16,
+                    # the FP16-entry kernel is VALU-bound with the same 7 lane-instructions per
+                    # update (interpolation in FP32): at 100 % of the issue rate it would run at
+                    # VALU_PEAK_T / 7 updates/s, which caps its ratio to this FP32 run
+                    "roofline_f16": {"bound": "valu", "achieved": VALU_PER_UPDATE * updates / bp16 / 1e12,
+                                     "peak": VALU_PEAK_T, "unit": "T lane-instr/s",
+                                     "frac": VALU_PER_UPDATE * updates / bp16 / 1e12 / VALU_PEAK_T,
+                                     "valu_per_update": VALU_PER_UPDATE,
+                                     "f16_over_f32_bp_ceiling": VALU_PEAK_T * 1e12 / VALU_PER_UPDATE / (updates / bp32)}})
     if e2e is not None:
         out["ms_e2e_from_pinned_host_f32"] = e2e[0] * 1e3
         out["ms_reconstruct_host_to_host_f32"] = h2h * 1e3
