@@ -41,12 +41,16 @@ namespace bp_detail {
 
 constexpr int kTX = 16, kTY = 16, kThreads = 256, kWaves = kThreads / 64;
 constexpr int kMaxTileBytes = 36 * 1024;  // per LDS buffer (two buffers per workgroup)
+// Detector pixels added around a footprint box's corner bounds: the corner and voxel
+// expressions round differently by about 1e-4 pixel at most.
+constexpr float kBoxMargin = 0.05f;
 
 struct BpArgs {
     int W, H, nx, ny, nzs, z0;
     float xpre, ypre, zpre, upre, vpre;
     float r2, fcd, uoff, ulo, uhi, vlo, vhi;
     int nproj, ntx, nty, nzb, cap, accumulate, vec4;
+    int blk, ntxp, ntyp;     // tile blocks of blk x blk (0: rows of tiles); tile counts rounded up to blk
     int zshift;              // z0 mod ZPL: runs start at grid z multiples of ZPL, zshift voxels before the slab
     int dma16;               // 8-byte entries staged with 16-byte DMA (H even: rows start even)
     int pitch_mod, pitch_rem;  // LDS column pitch = rows rounded up to pitch_rem mod pitch_mod (a power of two)
@@ -145,10 +149,25 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E::kMi
     const unsigned N = gridDim.x, L = blockIdx.x;
     const unsigned xcd = L & 7u, kk = L >> 3, q8 = N >> 3, r8 = N & 7u;
     const unsigned item = xcd * q8 + min(xcd, r8) + kk;
-    const unsigned ntxy = (unsigned)(a.ntx * a.nty);
-    const int zbi = (int)(item / ntxy);
-    const int txy = (int)(item - (unsigned)zbi * ntxy);
-    const int ty = txy / a.ntx, tx = txy - ty * a.ntx;
+    int zbi, tx, ty;
+    if (a.blk > 0) {
+        // consecutive items cover blk x blk squares of tiles, whose detector footprints
+        // overlap along every ray direction (the workgroups an XCD runs together then
+        // share their boxes in its L2); tiles of the rounded-up grid beyond it exit
+        const unsigned ntxy = (unsigned)(a.ntxp * a.ntyp), bb = (unsigned)(a.blk * a.blk);
+        zbi = (int)(item / ntxy);
+        const unsigned t = item - (unsigned)zbi * ntxy, b = t / bb, w = t - b * bb;
+        const unsigned sbx = (unsigned)(a.ntxp / a.blk), by = b / sbx;
+        tx = (int)((b - by * sbx) * a.blk + w % a.blk);
+        ty = (int)(by * a.blk + w / a.blk);
+        if (tx >= a.ntx || ty >= a.nty) return;
+    } else {
+        const unsigned ntxy = (unsigned)(a.ntx * a.nty);
+        zbi = (int)(item / ntxy);
+        const int txy = (int)(item - (unsigned)zbi * ntxy);
+        ty = txy / a.ntx;
+        tx = txy - ty * a.ntx;
+    }
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int xi = tx * kTX + (tid & (kTX - 1));
@@ -212,11 +231,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E::kMi
                 b.z = 0;
                 return b;
             }
-            const int us0 = (int)floorf(umin) - 1, vs0 = (int)floorf(vmin) - 1;
+            // A voxel reads the one entry at (floor(uIdx), floor(vIdx)).  uIdx and vIdx are
+            // projective in the voxel's position, so over the tile and run they lie between
+            // the corners' values; kBoxMargin covers the rounding of either expression.
+            const int us0 = (int)floorf(umin - kBoxMargin), vs0 = (int)floorf(vmin - kBoxMargin);
             const int us = max(us0, 0);
-            const int ue = min((int)floorf(umax) + 1, a.W - 1);
+            const int ue = min((int)floorf(umax + kBoxMargin), a.W - 1);
             const int vs = max(vs0, 0) & (a.dma16 ? ~1 : ~0);
-            const int ve = min((int)floorf(vmax) + 1, a.H - 1);
+            const int ve = min((int)floorf(vmax + kBoxMargin), a.H - 1);
             const int nc = ue - us + 1, nr = ve - vs + 1;
             if (nc <= 0 || nr <= 0) { b.z = 0; return b; }
             // LDS column pitch: rows rounded up to pitch_rem mod pitch_mod (a power of two,
@@ -246,12 +268,20 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E::kMi
             const dma_rsrc rs = make_dma_rsrc(reinterpret_cast<const char*>(proj) + (size_t)p * img_bytes, img_bytes);
             const int col0 = (us * a.H + vs) * E::kBytes;
             const unsigned lane_off = (unsigned)lane * ub;
-            for (int c = wave_u; c < nc; c += kWaves) {
-                const int gcol = col0 + c * a.H * E::kBytes;
-                unsigned char* lcol = dst + (size_t)c * pitch * E::kBytes;
-                for (int j = 0; j < units; j += 64) {
-                    if (lane < units - j) lds_dma<ub>(rs, lane_off, gcol + j * ub, lcol + (size_t)j * ub);
-                }
+            const int gstep = kWaves * a.H * E::kBytes, lstep = kWaves * pitch * E::kBytes;
+            // A DMA instruction costs about the same whatever its lane mask, and the loop
+            // around it is scalar work: every column takes `full` unmasked instructions,
+            // then the columns' partial tails share one lane mask, set once per box.
+            const int full = units >> 6, tail = units & 63;
+            for (int j = 0; j < full; ++j) {
+                int gcol = col0 + wave_u * a.H * E::kBytes + j * 64 * ub;
+                unsigned char* lcol = dst + (size_t)wave_u * pitch * E::kBytes + (size_t)j * 64 * ub;
+                for (int c = wave_u; c < nc; c += kWaves, gcol += gstep, lcol += lstep) lds_dma<ub>(rs, lane_off, gcol, lcol);
+            }
+            if (tail > 0 && lane < tail) {
+                int gcol = col0 + wave_u * a.H * E::kBytes + full * 64 * ub;
+                unsigned char* lcol = dst + (size_t)wave_u * pitch * E::kBytes + (size_t)full * 64 * ub;
+                for (int c = wave_u; c < nc; c += kWaves, gcol += gstep, lcol += lstep) lds_dma<ub>(rs, lane_off, gcol, lcol);
             }
         };
 
@@ -538,10 +568,10 @@ int sample_max_box(const ctg_geometry& g, const float4* hp, int nproj, int z0, i
                         vmin = std::min(vmin, std::min(va, vb)); vmax = std::max(vmax, std::max(va, vb));
                     }
                     if (bad) continue;
-                    const int us = std::max((int)std::floor(umin) - 1, 0);
-                    const int ue = std::min((int)std::floor(umax) + 1, g.det_width - 1);
-                    const int vs = std::max((int)std::floor(vmin) - 1, 0) & (dma16 ? ~1 : ~0);
-                    const int ve = std::min((int)std::floor(vmax) + 1, g.det_height - 1);
+                    const int us = std::max((int)std::floor(umin - kBoxMargin), 0);
+                    const int ue = std::min((int)std::floor(umax + kBoxMargin), g.det_width - 1);
+                    const int vs = std::max((int)std::floor(vmin - kBoxMargin), 0) & (dma16 ? ~1 : ~0);
+                    const int ve = std::min((int)std::floor(vmax + kBoxMargin), g.det_height - 1);
                     if (ue >= us && ve >= vs) {
                         const int nr = ve - vs + 1;
                         const int pitch = nr + ((prem - nr) & (pmod - 1));
@@ -614,7 +644,13 @@ ctg_status launch_zpl(const BpLaunch& L, BpArgs a, hipStream_t st) {
     cap &= ~7;  // keeps the second buffer and every column start 16-byte aligned
     a.cap = cap;
     const size_t lds = (size_t)2 * cap * ebytes;
-    const long long grid = (long long)a.ntx * a.nty * a.nzb;
+    // workgroups in 4 x 4 blocks of tiles (measured against rows of tiles and 8 x 8 and
+    // 16 x 16 blocks); CTG_BP_BLOCK=n sets the block edge, 0 for rows (tuning)
+    a.blk = 4;
+    if (const char* e = std::getenv("CTG_BP_BLOCK")) a.blk = std::max(0, std::min(64, std::atoi(e)));
+    a.ntxp = a.blk > 0 ? (a.ntx + a.blk - 1) / a.blk * a.blk : a.ntx;
+    a.ntyp = a.blk > 0 ? (a.nty + a.blk - 1) / a.blk * a.blk : a.nty;
+    const long long grid = (long long)a.ntxp * a.ntyp * a.nzb;
     if (grid > 0x7fffffffll) return fail(CTG_ERROR_UNSUPPORTED, "backprojection grid too large");
 #define CTG_BP_LAUNCH(E, DMA)                                                                           \
     hipLaunchKernelGGL((bp_kernel<ZPL, E, DMA>), dim3((unsigned)grid), dim3(kThreads), lds, st, a, L.d_params, \
