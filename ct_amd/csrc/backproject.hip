@@ -241,10 +241,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E::kMi
             const int ve = min((int)floorf(vmax + kBoxMargin), a.H - 1);
             const int nc = ue - us + 1, nr = ve - vs + 1;
             if (nc <= 0 || nr <= 0) { b.z = 0; return b; }
-            // LDS column pitch: rows rounded up to pitch_rem mod pitch_mod (a power of two,
-            // so a mask: a division here would raise the kernel's register peak), so that
-            // neighbouring detector columns start in different LDS bank groups
-            const int pitch = nr + ((a.pitch_rem - nr) & (a.pitch_mod - 1));
+            // LDS column pitch: the rows the DMA writes (an even count when 16-byte lanes
+            // carry two 8-byte entries) rounded up to pitch_rem mod pitch_mod (a power of
+            // two, so a mask: a division here would raise the kernel's register peak), so
+            // that neighbouring detector columns start in different LDS bank groups
+            const int wr = (DMA == 16 && E::kBytes == 8) ? (nr + 1) & ~1 : nr;
+            const int pitch = wr + ((a.pitch_rem - wr) & (a.pitch_mod - 1));
             b.x = us; b.y = vs; b.w = nr | (pitch << 16);
             // the fused vIndex of the fast path may round a hair below 0 at the top edge
             const int fast = vs0 >= 0 ? 1 : 0;
@@ -574,7 +576,8 @@ int sample_max_box(const ctg_geometry& g, const float4* hp, int nproj, int z0, i
                     const int ve = std::min((int)std::floor(vmax + kBoxMargin), g.det_height - 1);
                     if (ue >= us && ve >= vs) {
                         const int nr = ve - vs + 1;
-                        const int pitch = nr + ((prem - nr) & (pmod - 1));
+                        const int wr = dma16 ? (nr + 1) & ~1 : nr;
+                        const int pitch = wr + ((prem - wr) & (pmod - 1));
                         best = std::max(best, (ue - us + 1) * pitch);
                     }
                 }
@@ -619,19 +622,20 @@ ctg_status launch_zpl(const BpLaunch& L, BpArgs a, hipStream_t st) {
     // whole runs store 16-byte vectors when the slab starts on a run boundary
     a.vec4 = (a.zshift == 0 && L.nz % 4 == 0 && ((uintptr_t)L.d_volume % 16) == 0) ? 1 : 0;
     a.nzb = (L.nz + a.zshift + ZPL - 1) / ZPL;
-    // 8-byte entries are staged 4 bytes per lane by default: the 16-byte form needs an
-    // even LDS pitch, which measured slower (more bank conflicts on the 8-byte reads) than
-    // the odd pitch.  CTG_BP_DMA16=1 selects it (needs an even detector height).
-    a.dma16 = (ebytes == 8 && g.det_height % 2 == 0 && std::getenv("CTG_BP_DMA16") &&
-               std::atoi(std::getenv("CTG_BP_DMA16")) == 1) ? 1 : 0;
-    // column pitch: rows rounded up to 1 mod 16 (2 mod 16 for the 16-byte form of 8-byte
-    // entries).  CTG_BP_PITCH="mod,rem" overrides (tuning).
+    // 8-byte entries are staged 16 bytes (two rows) per lane when the detector height is
+    // even, so that the first row of every box can be even and each lane's 16 bytes are
+    // aligned in global memory: one DMA instruction per column of up to 128 rows instead
+    // of two beyond 32.  A DMA instruction costs about the same whatever it moves, and the
+    // LDS side needs only 8-byte alignment, so the odd pitch stays.  CTG_BP_DMA16=0
+    // selects 4 bytes per lane (tuning).
+    const char* dma_env = std::getenv("CTG_BP_DMA16");
+    a.dma16 = (ebytes == 8 && g.det_height % 2 == 0 && !(dma_env && std::atoi(dma_env) == 0)) ? 1 : 0;
+    // column pitch: rows rounded up to 1 mod 16.  CTG_BP_PITCH="mod,rem" overrides (tuning).
     a.pitch_mod = 16;
-    a.pitch_rem = a.dma16 ? 2 : 1;
+    a.pitch_rem = 1;
     if (const char* env = std::getenv("CTG_BP_PITCH")) {
         int m = 0, r = 0;
-        if (std::sscanf(env, "%d,%d", &m, &r) == 2 && m >= 1 && (m & (m - 1)) == 0 && r >= 0 &&
-            (!a.dma16 || (m % 2 == 0 && r % 2 == 0))) {
+        if (std::sscanf(env, "%d,%d", &m, &r) == 2 && m >= 1 && (m & (m - 1)) == 0 && r >= 0) {
             a.pitch_mod = m;
             a.pitch_rem = r % m;
         }
