@@ -32,14 +32,20 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, 8 TB/s
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "backproject_traffic_1024cubed_720proj.json")
 BYTES_PER_UPDATE = 8.0     # one FP32 voxel load + one store per (voxel, projection)
-# VALU issue peak, non-packed FP32: 256 CUs x 64 lanes x 2.4 GHz, in 1e12 lane-instructions/s
+# VALU issue peak at 4 cycles per wave64 instruction per SIMD: 256 CUs x 64 lanes x
+# 2.4 GHz, in 1e12 lane-instructions/s.  ct_amd/isa_rates measures 4.2-4.4 cycles for
+# v_fract_f32, v_cvt_flr_i32_f32, v_lshl_add_u32 and v_fma_mix_f32 but 2.8 for v_fma_f32,
+# so for an FMA-heavy mix this is a lower bound of the issue rate.
 VALU_PEAK_T = 256 * 64 * 2.4e9 / 1e12
+# MI355X's published FP32 vector peak (TFLOP/s) and the flops of one voxel-update
+FP32_PEAK_TFLOPS = 157.3
+FLOPS_PER_UPDATE = 8.0
 # lane-instructions per voxel-update of the register-accumulating formulation: one FMA
 # for the detector row, floor and fraction of it, three FMAs for the bilinear form and
 # one FMA into the accumulator (the LDS address is not counted)
 VALU_PER_UPDATE = 7.0
 # LDS read bandwidth: 256 bytes per clock per CU (a ds_read_b128 serves 16 lanes per
-# 4-cycle pass: ct_amd/lds_rates measures at most 6.6 cycles per conflict-free wave
+# 4-cycle pass: ct_amd/isa_rates measures at most 6.6 cycles per conflict-free wave
 # read and 16.9 for a 4-way conflict), 256 CUs at 2.4 GHz, in TB/s
 LDS_PEAK_TBS = 256 * 256 * 2.4e9 / 1e12
 
@@ -474,6 +480,12 @@ def main():
                      "frac": achieved / VALU_PEAK_T, "traffic": traffic, "compulsory_bytes": compulsory,
                      "traffic_over_compulsory": traffic / compulsory if traffic else None,
                      "valu_per_update": VALU_PER_UPDATE},
+        # against the published FP32 vector peak: the update's arithmetic is the three
+        # FMAs of the bilinear form and the accumulating FMA, 8 flops
+        "roofline_fp32_peak": {"bound": "fp32", "achieved": FLOPS_PER_UPDATE * updates / bp32 / 1e12,
+                               "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                               "frac": FLOPS_PER_UPDATE * updates / bp32 / 1e12 / FP32_PEAK_TFLOPS,
+                               "flops_per_update": FLOPS_PER_UPDATE},
         # the other on-chip ceiling: one 16-byte FP32 entry read from LDS per update,
         # against 256 B per clock per CU (256 CUs at 2.4 GHz)
         "roofline_lds": {"bound": "lds", "achieved": 16.0 * updates / bp32 / 1e12, "peak": LDS_PEAK_TBS,

@@ -1,8 +1,9 @@
-// lds_rates: measures the LDS instruction rates that bound the LDS-heavy kernels
-// (the entry-space forward projector's 16-byte read-modify-writes and the
-// backprojection's 16-byte entry reads).  Not used by the library.
+// isa_rates: measures the instruction rates that bound the library's kernels on gfx950:
+// LDS reads and writes (the entry-space forward projector's 16-byte read-modify-writes,
+// the backprojection's 16-byte entry reads) and the VALU instructions of the
+// backprojection's per-voxel arithmetic.  Not used by the library.
 //
-// Every workgroup (256 lanes, 64 KB of LDS: 16 KB per wave) issues `iters` x 8 LDS
+// LDS: every workgroup (256 lanes, 64 KB of LDS: 16 KB per wave) issues `iters` x 8 LDS
 // instructions of one kind per wave at addresses inside its own wave's region; the
 // kernel is launched twice and the second launch is timed with HIP events.  The result
 // is printed as device clock cycles per wave instruction per CU:
@@ -10,6 +11,11 @@
 // with the clock and CU count from the device attributes.  The arithmetic around each
 // access (address, one add) is a few VALU instructions per access, so a figure near 6
 // or below for reads is bounded by VALU issue, not by the LDS.
+//
+// VALU: 8 workgroups of 256 lanes per CU (8 waves per SIMD) run 8 independent register
+// chains per lane of one instruction kind (inline assembly on registers only, so the
+// compiler neither folds nor reorders them), 512 per loop iteration; the result is
+// device cycles per wave64 instruction per SIMD.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -78,6 +84,39 @@ __global__ __launch_bounds__(256) void lds_kernel(int iters, float* sink) {
     if (acc == -1.0f || base4[lane].x == -1.0f) sink[threadIdx.x] = acc;
 }
 
+enum Valu { kFma = 0, kFmaMix = 1, kFract = 2, kCvtFlr = 3, kLshlAdd = 4, kAddU32 = 5 };
+
+template <int KIND>
+__global__ __launch_bounds__(256) void valu_kernel(int iters, float a, float b, float* sink) {
+    float x[8];
+    int y[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        x[i] = (float)threadIdx.x * 0.001f + (float)i;
+        y[i] = (int)threadIdx.x + i;
+    }
+    const int ah = __float_as_int(a) & 0xffff;  // a 16-bit pattern for the mixed-precision form
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int s = 0; s < 64; ++s) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if constexpr (KIND == kFma) asm volatile("v_fma_f32 %0, %0, %1, %2" : "+v"(x[i]) : "v"(a), "v"(b));
+                else if constexpr (KIND == kFmaMix)
+                    asm volatile("v_fma_mix_f32 %0, %0, %1, %2 op_sel_hi:[0,1,0]" : "+v"(x[i]) : "v"(ah), "v"(b));
+                else if constexpr (KIND == kFract) asm volatile("v_fract_f32 %0, %0" : "+v"(x[i]));
+                else if constexpr (KIND == kCvtFlr) asm volatile("v_cvt_flr_i32_f32 %0, %1" : "=v"(y[i]) : "v"(x[i]));
+                else if constexpr (KIND == kLshlAdd) asm volatile("v_lshl_add_u32 %0, %0, 4, %1" : "+v"(y[i]) : "v"(ah));
+                else asm volatile("v_add_u32 %0, %0, %1" : "+v"(y[i]) : "v"(ah));
+            }
+        }
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s += x[i] + (float)y[i];
+    if (s == -1.0f) sink[threadIdx.x] = s;
+}
+
 }  // namespace
 
 int main() {
@@ -116,6 +155,29 @@ int main() {
         float ms = 0.0f;
         LR_CHECK(hipEventElapsedTime(&ms, t0, t1));
         std::printf("%6.2f  %s\n", ms * 1e-3 * clock_khz * 1e3 * cus / instr, c.what);
+        std::fflush(stdout);
+    }
+    const int vblocks = 8 * cus, viters = 200;
+    const double vinstr = (double)vblocks * 4 * viters * 512;
+    std::printf("cycles per wave64 instruction per SIMD (8 waves per SIMD)\n");
+    using VKernel = void (*)(int, float, float, float*);
+    struct VCase { VKernel k; const char* what; };
+    const VCase vcases[] = {
+        {valu_kernel<kFma>, "v_fma_f32"},           {valu_kernel<kFmaMix>, "v_fma_mix_f32 (one 16-bit operand)"},
+        {valu_kernel<kFract>, "v_fract_f32"},       {valu_kernel<kCvtFlr>, "v_cvt_flr_i32_f32"},
+        {valu_kernel<kLshlAdd>, "v_lshl_add_u32"}, {valu_kernel<kAddU32>, "v_add_u32"},
+    };
+    for (const VCase& c : vcases) {
+        for (int rep = 0; rep < 2; ++rep) {
+            if (rep == 1) LR_CHECK(hipEventRecord(t0));
+            hipLaunchKernelGGL(c.k, dim3(vblocks), dim3(256), 0, 0, viters, 0.999f, 0.001f, sink);
+            LR_CHECK(hipGetLastError());
+        }
+        LR_CHECK(hipEventRecord(t1));
+        LR_CHECK(hipEventSynchronize(t1));
+        float ms = 0.0f;
+        LR_CHECK(hipEventElapsedTime(&ms, t0, t1));
+        std::printf("%6.2f  %s\n", ms * 1e-3 * clock_khz * 1e3 * cus * 4 / vinstr, c.what);
         std::fflush(stdout);
     }
     LR_CHECK(hipFree(sink));
