@@ -48,6 +48,10 @@ VALU_PER_UPDATE = 7.0
 # 4-cycle pass: ct_amd/isa_rates measures at most 6.6 cycles per conflict-free wave
 # read and 16.9 for a 4-way conflict), 256 CUs at 2.4 GHz, in TB/s
 LDS_PEAK_TBS = 256 * 256 * 2.4e9 / 1e12
+# the same at the measured conflict-free ds_read_b128 rate: 1 KB per wave instruction
+# every 6.64 cycles per CU (ct_amd/isa_rates)
+DS_READ_B128_CYCLES = 6.64
+LDS_READ_MEASURED_TBS = 256 * (64 * 16 / DS_READ_B128_CYCLES) * 2.4e9 / 1e12
 
 
 def log(msg):
@@ -487,9 +491,12 @@ def main():
                                "frac": FLOPS_PER_UPDATE * updates / bp32 / 1e12 / FP32_PEAK_TFLOPS,
                                "flops_per_update": FLOPS_PER_UPDATE},
         # the other on-chip ceiling: one 16-byte FP32 entry read from LDS per update,
-        # against 256 B per clock per CU (256 CUs at 2.4 GHz)
+        # against 256 B per clock per CU (256 CUs at 2.4 GHz), and against the measured
+        # rate of conflict-free 16-byte LDS reads
         "roofline_lds": {"bound": "lds", "achieved": 16.0 * updates / bp32 / 1e12, "peak": LDS_PEAK_TBS,
-                         "unit": "TB/s", "frac": 16.0 * updates / bp32 / 1e12 / LDS_PEAK_TBS, "bytes_per_update": 16},
+                         "unit": "TB/s", "frac": 16.0 * updates / bp32 / 1e12 / LDS_PEAK_TBS, "bytes_per_update": 16,
+                         "peak_measured_ds_read_b128": LDS_READ_MEASURED_TBS,
+                         "frac_of_measured": 16.0 * updates / bp32 / 1e12 / LDS_READ_MEASURED_TBS},
         # the 8-byte read-modify-write model of the metric's definition (the reference
         # kernel's per-update voxel load and store, Ct/cuda_ct.cu:193-199): 1000 GUPS
         # at 8 TB/s, against which a 60 % HBM-roofline target reads 600 GUPS
