@@ -44,7 +44,7 @@ def _phantom_raw(ctx, torch, g, angles, ell, photons, seed):
 
 
 @pytest.fixture(scope="module")
-def cfg1(gpu):
+def scan256(gpu):
     import torch
     import ct_amd
     from ct_amd.phantom import circular_angles, phantom_unit_radius, random_ellipsoids
@@ -59,8 +59,8 @@ def cfg1(gpu):
     return dict(W=W, H=H, P=P, fcd=fcd, angles=np.asarray(angles), raw=raw)
 
 
-def test_cfg1_cli_zfastest_matches_oracle(cfg1, oracle, tmp_path):
-    s = cfg1
+def test_scan256_cli_zfastest_matches_oracle(scan256, oracle, tmp_path):
+    s = scan256
     pixel = 0.4
     csv = write_scan(str(tmp_path), s["raw"], s["angles"], pixel_size=pixel, fcd_px=s["fcd"], base=1.0)
     r = subprocess.run([CLI, "-i", csv, "-o", "vol.raw", "-j", "zfastest", "-f", "ramlak"], cwd=str(tmp_path),
@@ -78,10 +78,10 @@ def test_cfg1_cli_zfastest_matches_oracle(cfg1, oracle, tmp_path):
     assert err < 1e-4
 
 
-def test_cfg1_abi_256_cube_matches_oracle(cfg1, oracle):
+def test_scan256_abi_256_cube_matches_oracle(scan256, oracle):
     import torch
     import ct_amd
-    s = cfg1
+    s = scan256
     P = s["P"]
     g = ct_amd.geometry_grid(s["W"], s["H"], s["fcd"], 0.0, 256, 256, 256)
     raw = torch.from_numpy(s["raw"].view(np.int16)).cuda()
@@ -109,7 +109,7 @@ def test_cfg1_abi_256_cube_matches_oracle(cfg1, oracle):
 # ------------------------------------------------------------------ 1024^3 from 720 x 1024^2
 
 @pytest.fixture(scope="module")
-def cfg2(gpu, oracle):
+def scan1024(gpu, oracle):
     import torch
     import ct_amd
     sys.path.insert(0, ROOT)
@@ -148,9 +148,9 @@ def _oracle_slab(oracle, g, filtered, angles, z0, nz):
     return oracle.backproject_f32(sub, filtered, angles)
 
 
-def test_cfg2_central_slab_matches_oracle(cfg2, oracle):
+def test_scan1024_central_slab_matches_oracle(scan1024, oracle):
     import torch
-    c = cfg2
+    c = scan1024
     g, P = c["g"], len(c["angles"])
     vol = _gpu_fdk(c["ctx"], torch, g, c["raw"], P, c["base"], "f32")
     z0, nz = g.nz // 2 - 32, 64
@@ -162,10 +162,10 @@ def test_cfg2_central_slab_matches_oracle(cfg2, oracle):
     assert err < 1e-4
 
 
-def test_cfg2_roi_matches_oracle(cfg2, oracle):
+def test_scan1024_roi_matches_oracle(scan1024, oracle):
     import torch
     import ct_amd
-    c = cfg2
+    c = scan1024
     g0, P = c["g"], len(c["angles"])
     g, roi = ct_amd.geometry_ctvolume(g0.det_width, g0.det_height, g0.fcd, 0.0, [0.25, 0.75, 0.25, 0.75, 0.1, 0.9])
     assert (g.nx, g.ny, g.nz) == (501, 501, 820) and tuple(roi) == (251, 752, 251, 752, 102, 922)
@@ -181,9 +181,9 @@ def test_cfg2_roi_matches_oracle(cfg2, oracle):
     c["ctx"].set_geometry(g0)
 
 
-def test_cfg2_fp32_and_fp16_against_fp64_all_voxels(cfg2):
+def test_scan1024_fp32_and_fp16_against_fp64_all_voxels(scan1024):
     import torch
-    c = cfg2
+    c = scan1024
     g, P = c["g"], len(c["angles"])
     ctx = c["ctx"]
     ctx.set_geometry(g)

@@ -222,6 +222,24 @@ def test_linearity_and_f32_vs_f64(oracle):
     assert rel_rmse(c, a) < 1e-5
 
 
+def test_backprojection_in_z_slabs_equals_the_whole_grid(oracle):
+    """The full-grid CPU baseline backprojects in z-slabs (each slab a grid of nz' slices
+    with zPre shifted by its first slice): every voxel's arithmetic is the same, so the
+    slabs put together equal the whole grid bit for bit."""
+    scan = small_scan(W=64, H=48, P=20, seed=9)
+    _, g, _, _ = oracle.volume_geometry(64, 48, scan["fcd"])
+    q = oracle.preprocess_f32(g, scan["raw"], "u16", 1.0, "shepplogan")
+    whole = oracle.backproject_f32(g, q, scan["angles"])
+    parts = []
+    for z0 in range(0, g.nz, 20):
+        sub = oracle.geometry_from(g)
+        sub.nz = min(20, g.nz - z0)
+        sub.z_pre = g.z_pre - z0
+        parts.append(oracle.backproject_f32(sub, q, scan["angles"]))
+    assert np.array_equal(np.concatenate(parts, axis=2), whole)
+    assert float(np.abs(whole).max()) > 0.0
+
+
 # --------------------------------------------------------------------- phantom
 
 def test_phantom_sphere_chords(oracle):
