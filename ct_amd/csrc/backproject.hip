@@ -54,6 +54,7 @@ struct BpArgs {
     int zshift;              // z0 mod ZPL: runs start at grid z multiples of ZPL, zshift voxels before the slab
     int dma16;               // 8-byte entries staged with 16-byte DMA (H even: rows start even)
     int pitch_mod, pitch_rem;  // LDS column pitch = rows rounded up to pitch_rem mod pitch_mod (a power of two)
+    int pitch_adapt;           // per box, -pitch_rem mod pitch_mod where the footprint slants the other way
 };
 
 // Entry formats: FP32 (16 B), FP16 and BF16 (8 B: {A,dU} {dV,dUV} as 16-bit pairs).
@@ -208,6 +209,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E::kMi
         auto box_for = [&](int p) -> int4 {
             const float4 g = pp[p];
             float umin = INFINITY, umax = -INFINITY, vmin = INFINITY, vmax = -INFINITY;
+            // changes of uIdx and vIdx along x (the lanes of a 16-lane row), summed over
+            // the tile's two y edges and the run's two z ends
+            float du = 0.0f, dv = 0.0f;
             bool bad = false;
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
@@ -224,6 +228,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E::kMi
                 const float vb = fmaf(-(zw1 + g.z), U, a.vpre);
                 umin = fminf(umin, ui); umax = fmaxf(umax, ui);
                 vmin = fminf(vmin, fminf(va, vb)); vmax = fmaxf(vmax, fmaxf(va, vb));
+                du += (c & 1) ? ui : -ui;
+                dv += (c & 1) ? va + vb : -(va + vb);
             }
             int4 b = make_int4(0, 0, 2 << 28, 0);
             if (bad || !(umin == umin) || !(vmin == vmin)) return b;
@@ -246,7 +252,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E::kMi
             // two, so a mask: a division here would raise the kernel's register peak), so
             // that neighbouring detector columns start in different LDS bank groups
             const int wr = (DMA == 16 && E::kBytes == 8) ? (nr + 1) & ~1 : nr;
-            const int pitch = wr + ((a.pitch_rem - wr) & (a.pitch_mod - 1));
+            int pitch = wr + ((a.pitch_rem - wr) & (a.pitch_mod - 1));
+            // A 16-lane row of voxels reads entries whose column and row both drift along
+            // x.  With the pitch at +1 mod 16 a lane's bank group is column + row, and two
+            // lanes whose column step is undone by a row step share one; where the column
+            // and row drift in opposite directions the pitch -1 mod 16 (row - column)
+            // avoids that.  Taken only where the box still fits the buffer.
+            if (a.pitch_adapt && du * dv < 0.0f) {
+                const int pm = wr + ((a.pitch_mod - a.pitch_rem - wr) & (a.pitch_mod - 1));
+                if (nc * pm <= a.cap) pitch = pm;
+            }
             b.x = us; b.y = vs; b.w = nr | (pitch << 16);
             // the fused vIndex of the fast path may round a hair below 0 at the top edge
             const int fast = vs0 >= 0 ? 1 : 0;
@@ -633,6 +648,11 @@ ctg_status launch_zpl(const BpLaunch& L, BpArgs a, hipStream_t st) {
     // column pitch: rows rounded up to 1 mod 16.  CTG_BP_PITCH="mod,rem" overrides (tuning).
     a.pitch_mod = 16;
     a.pitch_rem = 1;
+    // per-box choice of +1 or -1 mod 16 for 16-byte entries (FP32 179.4 -> 174.8 ms at
+    // 1024^3 x 720; no change for 8-byte entries, which keep +1).  CTG_BP_PITCH_ADAPT=0/1
+    // overrides (tuning).
+    a.pitch_adapt = ebytes == 16 ? 1 : 0;
+    if (const char* env = std::getenv("CTG_BP_PITCH_ADAPT")) a.pitch_adapt = std::atoi(env) != 0;
     if (const char* env = std::getenv("CTG_BP_PITCH")) {
         int m = 0, r = 0;
         if (std::sscanf(env, "%d,%d", &m, &r) == 2 && m >= 1 && (m & (m - 1)) == 0 && r >= 0) {
