@@ -289,6 +289,33 @@ def test_wide_cone_uses_global_path(gpu, oracle):
     assert err < FDK_TOL
 
 
+@pytest.mark.parametrize("store", ["f32", "f16"])
+def test_lds_layout_choices_do_not_change_the_volume(gpu, store):
+    """The LDS column pitch (+1 mod 16, or -1 per box where the footprint slants the other
+    way), the order of tiles (4 x 4 blocks, rows, 2 x 2 blocks) and the DMA width of
+    8-byte entries only move where entries sit in LDS and when workgroups run: every
+    voxel's sum is the same, bit for bit.  A short FCD (1.5 detector widths) slants the
+    footprints strongly."""
+    import os
+    import torch
+    scan = small_scan(W=128, H=96, P=40, fcd=192.0, seed=21)
+    ctx, g = _ctx(scan)
+    ref, filtered, scales = _gpu_fdk(ctx, g, scan, store=store)
+    assert float(np.abs(ref).max()) > 0.0
+    cases = [("CTG_BP_PITCH_ADAPT", "0"), ("CTG_BP_PITCH_ADAPT", "1"), ("CTG_BP_BLOCK", "0"), ("CTG_BP_BLOCK", "2")]
+    if store == "f16":
+        cases.append(("CTG_BP_DMA16", "0"))
+    vol = torch.empty((g.nx, g.ny, g.nz), dtype=torch.float32, device="cuda")
+    for key, val in cases:
+        os.environ[key] = val
+        try:
+            ctx.backproject(filtered, store, scales, 0, scan["P"], vol, 0, g.nz)
+            torch.cuda.synchronize()
+        finally:
+            del os.environ[key]
+        assert np.array_equal(vol.cpu().numpy(), ref), (store, key, val)
+
+
 def test_slabs_and_projection_batches_compose(gpu):
     import torch
     scan = small_scan(W=64, H=64, P=24, seed=2)
