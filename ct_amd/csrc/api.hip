@@ -835,14 +835,18 @@ ctg_status ctg_reconstruct(ctg_context* ctx, const void* h_raw, ctg_pixel_type p
     // Z_FASTEST host volumes ([x][y][z]) take a cut planned along z as twice as many x
     // ranges (whole 16-column tiles) instead: an x range is contiguous in the host
     // volume, so its download is one sequential copy rather than a strided one per
-    // column, and the smaller last range leaves less download after the last
-    // backprojection (1024^3: 300 ms host to host against 310 ms for four ranges).  The backprojection of a range is the one of the whole grid restricted to
-    // it (a grid of its columns with x_pre shifted by its start): same voxels, same
-    // tiles, same results.
+    // column.  Every range's download runs under the next range's backprojection except
+    // the last.  1024^3 from 720 x 1024^2 is planned as four z-slabs, so eight ranges:
+    // 206.6 ms host to host, against 228.8 / 212.7 / 212.9 ms for 2 / 4 / 16 ranges
+    // (CTG_RECON_XPARTS=n sets the count, tuning).  The backprojection of a range is the
+    // one of the whole grid restricted to it (a grid of its columns with x_pre shifted by
+    // its start): same voxels, same tiles, same results.
     const int nslabs_z = (g.nz + slab_nz - 1) / slab_nz;
     int part_x = g.nx;
     if (dbl && order == CTG_Z_FASTEST) {
-        part_x = ((g.nx + 2 * nslabs_z - 1) / (2 * nslabs_z) + 15) / 16 * 16;
+        int parts = 2 * nslabs_z;
+        if (const char* e = std::getenv("CTG_RECON_XPARTS")) parts = std::max(1, std::atoi(e));
+        part_x = ((g.nx + parts - 1) / parts + 15) / 16 * 16;
         if ((size_t)part_x * g.ny * g.nz > col * (size_t)slab_nz) part_x = g.nx;  // no larger than the planned slab
     }
     const bool xcut = part_x < g.nx;
