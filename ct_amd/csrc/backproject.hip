@@ -14,11 +14,12 @@
 //    column-major, so one voxel-update is one LDS read of one entry and
 //    val = A + fu*dU + fv*(dV + fu*dUV): 3 FMAs, plus 1 FMA into the accumulator;
 //  - a workgroup (256 lanes = 16x16 columns) stages, per projection, the bounding box of
-//    its voxel block's detector footprint into LDS with global->LDS DMA
-//    (global_load_lds), one detector column per DMA burst, double-buffered so that the
-//    DMA of projection p+1 runs under the compute of p (the two buffers reach the
-//    compute as __restrict__ pointers, without which the compiler waits for the DMA
-//    before the first LDS read);
+//    its voxel block's detector footprint into LDS with global->LDS DMA (buffer loads
+//    to LDS), one detector column per DMA burst, double-buffered so that the DMA of
+//    projection p+1 runs under the compute of p (the two buffers reach the compute as
+//    __restrict__ pointers, without which the compiler waits for the DMA before the
+//    first LDS read); a box's detector columns sit at an LDS pitch of +1 or -1 mod 16
+//    entries, chosen per box against bank conflicts;
 //  - per z the lane does one FMA for vIndex, v_cvt_flr/v_fract, one address op, the LDS
 //    read and 4 FMAs, with the reads software-pipelined in small groups; the
 //    per-column geometry (one reciprocal) is amortised over the run;
@@ -26,7 +27,8 @@
 //    a projection whose box exceeds the tile capacity reads entries from global memory
 //    instead, so any geometry stays correct;
 //  - workgroups are numbered so that each XCD (dispatch is round-robin over 8 XCDs)
-//    receives a contiguous range of tiles and shares detector rows in its own L2.
+//    receives a contiguous range of tiles, in 4x4 blocks of tiles, and shares detector
+//    footprints in its own L2.
 #include "ctg_internal.h"
 
 #include <hip/hip_fp16.h>
