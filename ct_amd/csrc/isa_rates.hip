@@ -84,7 +84,10 @@ __global__ __launch_bounds__(256) void lds_kernel(int iters, float* sink) {
     if (acc == -1.0f || base4[lane].x == -1.0f) sink[threadIdx.x] = acc;
 }
 
-enum Valu { kFma = 0, kFmaMix = 1, kFract = 2, kCvtFlr = 3, kLshlAdd = 4, kAddU32 = 5 };
+enum Valu {
+    kFma = 0, kFmaMix = 1, kFract = 2, kCvtFlr = 3, kLshlAdd = 4, kAddU32 = 5,
+    kLshl = 6, kMadU24 = 7, kAddF32 = 8, kMulF32 = 9, kCvtF32I32 = 10, kAndB32 = 11
+};
 
 template <int KIND>
 __global__ __launch_bounds__(256) void valu_kernel(int iters, float a, float b, float* sink) {
@@ -107,6 +110,12 @@ __global__ __launch_bounds__(256) void valu_kernel(int iters, float a, float b, 
                 else if constexpr (KIND == kFract) asm volatile("v_fract_f32 %0, %0" : "+v"(x[i]));
                 else if constexpr (KIND == kCvtFlr) asm volatile("v_cvt_flr_i32_f32 %0, %1" : "=v"(y[i]) : "v"(x[i]));
                 else if constexpr (KIND == kLshlAdd) asm volatile("v_lshl_add_u32 %0, %0, 4, %1" : "+v"(y[i]) : "v"(ah));
+                else if constexpr (KIND == kLshl) asm volatile("v_lshlrev_b32 %0, 3, %0" : "+v"(y[i]));
+                else if constexpr (KIND == kMadU24) asm volatile("v_mad_u32_u24 %0, %0, 8, %1" : "+v"(y[i]) : "v"(ah));
+                else if constexpr (KIND == kAddF32) asm volatile("v_add_f32 %0, %0, %1" : "+v"(x[i]) : "v"(b));
+                else if constexpr (KIND == kMulF32) asm volatile("v_mul_f32 %0, %0, %1" : "+v"(x[i]) : "v"(a));
+                else if constexpr (KIND == kCvtF32I32) asm volatile("v_cvt_f32_i32 %0, %1" : "=v"(x[i]) : "v"(y[i]));
+                else if constexpr (KIND == kAndB32) asm volatile("v_and_b32 %0, %0, %1" : "+v"(y[i]) : "v"(ah));
                 else asm volatile("v_add_u32 %0, %0, %1" : "+v"(y[i]) : "v"(ah));
             }
         }
@@ -166,6 +175,9 @@ int main() {
         {valu_kernel<kFma>, "v_fma_f32"},           {valu_kernel<kFmaMix>, "v_fma_mix_f32 (one 16-bit operand)"},
         {valu_kernel<kFract>, "v_fract_f32"},       {valu_kernel<kCvtFlr>, "v_cvt_flr_i32_f32"},
         {valu_kernel<kLshlAdd>, "v_lshl_add_u32"}, {valu_kernel<kAddU32>, "v_add_u32"},
+        {valu_kernel<kLshl>, "v_lshlrev_b32"},      {valu_kernel<kMadU24>, "v_mad_u32_u24"},
+        {valu_kernel<kAddF32>, "v_add_f32"},        {valu_kernel<kMulF32>, "v_mul_f32"},
+        {valu_kernel<kCvtF32I32>, "v_cvt_f32_i32"}, {valu_kernel<kAndB32>, "v_and_b32"},
     };
     for (const VCase& c : vcases) {
         for (int rep = 0; rep < 2; ++rep) {
