@@ -742,6 +742,39 @@ def test_reconstruct_image_depths_and_stores(gpu, pixel, store):
     assert err < 1e-6
 
 
+def test_reconstruct_x_ranges_give_the_same_volume(gpu):
+    """A resident Z_FASTEST volume of 256 or more slices is planned as four z-slabs and
+    reconstructed as x ranges of whole 16-column tiles (two per planned slab by default,
+    CTG_RECON_XPARTS to change the count), each a grid of its own columns.  Five ranges
+    of 32 columns, ten of 16, and no cut (the four z-slabs) give the same volume bit for
+    bit."""
+    import os
+    import ct_amd
+    from ct_amd.phantom import circular_angles
+    W, H, P, n, nz = 192, 64, 24, 160, 256
+    g = ct_amd.geometry_grid(W, H, 600.0, 0.0, n, n, nz)
+    ctx = ct_amd.Context(0)
+    ctx.set_geometry(g)
+    ctx.set_projections(circular_angles(P))
+    raw = np.random.default_rng(8).integers(2000, 60000, size=(P, H, W), dtype=np.uint16)
+    assert ctx.plan_reconstruct("u16", P, "f32") == (True, 64)
+
+    def recon(parts):
+        out = np.empty(n * n * nz, np.float32)
+        if parts is not None:
+            os.environ["CTG_RECON_XPARTS"] = str(parts)
+        try:
+            ctx.reconstruct(raw.ctypes.data, "u16", P, 1.0, "cpu", "ramlak", "f32", out.ctypes.data, "zfastest")
+        finally:
+            os.environ.pop("CTG_RECON_XPARTS", None)
+        return out
+
+    ref = recon(None)
+    assert float(np.abs(ref).max()) > 0.0
+    for parts in (10, 1):
+        assert np.array_equal(recon(parts), ref), parts
+
+
 def test_reconstruct_memory_plans_slabs_and_recomputation(gpu):
     """ctg_reconstruct under memory budgets: resident entries with several slabs give the
     same volume bit for bit; entries re-derived per slab and batch (non-resident) give it
