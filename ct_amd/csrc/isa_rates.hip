@@ -1,7 +1,8 @@
 // isa_rates: measures the instruction rates that bound the library's kernels on gfx950:
 // LDS reads and writes (the entry-space forward projector's 16-byte read-modify-writes,
-// the backprojection's 16-byte entry reads) and the VALU instructions of the
-// backprojection's per-voxel arithmetic.  Not used by the library.
+// the backprojection's 16-byte entry reads) and VALU instructions: those of the
+// backprojection's per-voxel arithmetic and the alternatives weighed for it (FP32
+// add/mul, integer add/and, shifts, conversions).  Not used by the library.
 //
 // LDS: every workgroup (256 lanes, 64 KB of LDS: 16 KB per wave) issues `iters` x 8 LDS
 // instructions of one kind per wave at addresses inside its own wave's region; the
